@@ -1,0 +1,179 @@
+/*
+ * tsgpu — C ABI of the MI355X (gfx950) patch-search engine for
+ * example-based texture synthesis.
+ *
+ * The boundary replaces, inside EmbarkStudios/texture-synthesis (lib/src),
+ *   Generator::resolve_random_batch        ms.rs:427-445
+ *   Generator::resolve                     ms.rs:702-1051
+ *   ImagePyramid::new / build_gaussian     img_pyramid.rs:7-37
+ * as called by Session::run (session.rs:37-66) and SessionBuilder::build
+ * (session.rs:336-448). All entry points take plain pointers and sizes;
+ * images are row-major RGBA8 with stride 4*w (image::RgbaImage layout).
+ * The caller owns every host buffer; buffers are copied during the call.
+ * The library owns all device memory; handles are opaque.
+ *
+ * Status codes: 0 on success, negative on failure; tsg_last_error() returns
+ * a thread-local message for the last failure on the calling thread.
+ */
+#ifndef TSGPU_H
+#define TSGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes. -1..-5 mirror texture_synthesis::Error (errors.rs:38-57). */
+enum tsg_status {
+    TSG_OK = 0,
+    TSG_ERR_INVALID_RANGE = -1,          /* Error::InvalidRange */
+    TSG_ERR_NO_EXAMPLES = -2,            /* Error::NoExamples */
+    TSG_ERR_EXAMPLE_GUIDE_MISMATCH = -3, /* Error::ExampleGuideMismatch */
+    TSG_ERR_SIZE_MISMATCH = -4,          /* Error::SizeMismatch */
+    TSG_ERR_MAPS_COUNT_MISMATCH = -5,    /* Error::MapsCountMismatch */
+    TSG_ERR_INVALID_ARGUMENT = -10,      /* null pointer, zero size, bad index */
+    TSG_ERR_UNSUPPORTED = -11,           /* outside the device path's limits */
+    TSG_ERR_DEVICE = -20,                /* HIP runtime failure */
+    TSG_ERR_OUT_OF_MEMORY = -21,         /* device allocation failed */
+    TSG_ERR_SAMPLING = -22               /* a sample mask admits no pixel */
+};
+
+/* GeneratorParams (ms.rs:17-42) lowered from Parameters (lib.rs:327-375).
+ * tsg_params_default() gives the defaults of lib.rs:342-359. */
+typedef struct tsg_params {
+    uint32_t nearest_neighbors;       /* k, 1..64 on the device. default 50 */
+    uint64_t random_sample_locations; /* m >= 1. default 50 */
+    float cauchy_dispersion;          /* [0,1]. default 1.0 */
+    float backtrack_percent;          /* p in [0,1]. default 0.5 */
+    uint32_t backtrack_stages;        /* P. default 5 */
+    float guide_alpha;                /* [0,1]. default 0.8 */
+    uint64_t seed;                    /* default 0 */
+    uint32_t tiling_mode;             /* 0/1. default 0 */
+    uint64_t random_init;             /* UINT64_MAX = none (random_resolve None) */
+    uint32_t max_thread_count;        /* accepted for API parity (>= 1); the device ignores it */
+} tsg_params;
+
+/* Jacobi batch schedule of the device path. All items of a batch read the
+ * state as it stood at the batch start. px_per_batch > 0 fixes the batch
+ * size (1 reproduces the reference at max_thread_count(1)); otherwise a
+ * redo batch holds max(1, (resolved - locked) >> redo_shift) items and a
+ * new-pixel batch max(1, (resolved - locked) >> new_shift), both capped at
+ * max_batch when nonzero. */
+typedef struct tsg_schedule {
+    uint32_t px_per_batch;
+    uint32_t redo_shift;
+    uint32_t new_shift;
+    uint32_t max_batch;
+} tsg_schedule;
+
+typedef struct tsg_image {
+    uint32_t w, h;
+    const uint8_t* rgba; /* w*h*4 bytes */
+} tsg_image;
+
+/* SamplingMethod (lib.rs:458-470) */
+enum tsg_sample_kind { TSG_SAMPLE_ALL = 0, TSG_SAMPLE_IGNORE = 1, TSG_SAMPLE_IMAGE = 2 };
+
+typedef struct tsg_ctx tsg_ctx;
+typedef struct tsg_job tsg_job;
+
+/* Progress, as GeneratorProgress::update (session.rs:527-558): called on the
+ * caller's thread between batches whenever the rounded overall percentage
+ * changes. `rgba` (out_w*out_h*4) is valid only during the call. */
+typedef void (*tsg_progress_fn)(void* user, const uint8_t* rgba, uint32_t w, uint32_t h, uint64_t total_current,
+                                uint64_t total, uint64_t stage_current, uint64_t stage_total);
+
+void tsg_params_default(tsg_params* p);
+void tsg_schedule_default(tsg_schedule* s);
+const char* tsg_last_error(void);
+const char* tsg_version(void);
+
+/* Context: one HIP device and stream. */
+int tsg_ctx_create(int device, tsg_ctx** out);
+void tsg_ctx_destroy(tsg_ctx* ctx);
+
+/* SessionBuilder::build (session.rs:336-448) for already-decoded images.
+ * examples: finest example images at their final size, in add order
+ *           (load_image / resize_input are the caller's, utils.rs:55-80).
+ * sample_kind[i] / sample_masks[i]: per example; sample_kind NULL = all ALL.
+ *           A TSG_SAMPLE_IMAGE mask has the example's size.
+ * example_guides: NULL, or one guide per example (all-or-none,
+ *           session.rs:514-521); requires target_guide.
+ * target_guide: NULL, or an out_w x out_h guide.
+ * inpaint_mask: NULL, or an out_w x out_h mask; pixels with R < 255 are
+ *           synthesised (ms.rs:271-279); examples[inpaint_index] is then the
+ *           initial colour map and must be out_w x out_h.
+ * Validation errors are returned before any device work. */
+int tsg_job_create(tsg_ctx* ctx, const tsg_params* params, uint32_t n_examples, const tsg_image* examples,
+                   const int* sample_kind, const tsg_image* sample_masks, const tsg_image* example_guides,
+                   const tsg_image* target_guide, const tsg_image* inpaint_mask, uint32_t inpaint_index,
+                   uint32_t out_w, uint32_t out_h, tsg_job** out);
+/* Overrides the default Jacobi schedule (before tsg_job_run). */
+int tsg_job_set_schedule(tsg_job* job, const tsg_schedule* s);
+/* Session::run (session.rs:37-66): random init, then every stage. */
+int tsg_job_run(tsg_job* job, tsg_progress_fn progress, void* user);
+/* GeneratedImage accessors (lib.rs:378-455, ms.rs:605-684). Any pointer may
+ * be NULL. rgba: w*h*4. coord: w*h*3 u32 (x, y, map) — the .xform body.
+ * ids: w*h*2 u32 (patch id, map id). resolved_flat / resolved_score: the
+ * resolved list in order (w*h entries). */
+int tsg_job_download(tsg_job* job, uint8_t* rgba, uint32_t* coord, uint32_t* ids, uint32_t* resolved_flat,
+                     float* resolved_score);
+/* Per stage (index 0 = p_stage P): sum of the winning costs of
+ * pattern-matched items, their count, and items processed. Returns the
+ * number of stages written (<= cap). */
+int tsg_job_stage_stats(tsg_job* job, double* sums, uint64_t* counts, uint64_t* items, uint32_t cap);
+/* Device time of the last tsg_job_run in milliseconds, and batch count. */
+int tsg_job_timing(tsg_job* job, double* run_ms, uint64_t* n_batches);
+/* Per-kernel device timing of the next tsg_job_run (HIP events on the job's
+ * stream around every launch). */
+int tsg_job_set_profiling(tsg_job* job, int enable);
+/* After a profiled run, per kernel [resolve, commit, random candidates,
+ * recolour, stats]: summed milliseconds, items processed, launches.
+ * Each array holds 5 entries. */
+int tsg_job_kernel_stats(tsg_job* job, double* ms, uint64_t* items, uint64_t* launches);
+void tsg_job_destroy(tsg_job* job);
+
+/* ---- test and tooling entry points (no counterpart in the reference API;
+ *      they expose single steps of the path for parity tests) ---- */
+
+/* ImagePyramid::new(img, Some(levels)) on the device: writes max(levels,1)
+ * full-size RGBA8 levels, coarsest first (img_pyramid.rs:20-37). */
+int tsg_pyramid_build(tsg_ctx* ctx, const tsg_image* img, uint32_t levels, uint8_t* out);
+
+/* k nearest resolved neighbours (canonical order: d^2, dy, dx) of each
+ * query pixel against a 1-byte-per-pixel mask (nonzero = resolved),
+ * non-tiling. nbrs: n_q*k (x, y) int32 pairs; counts: n_q. Replaces
+ * TreeGrid::get_k_nearest_neighbors (ms.rs:1363-1530). */
+int tsg_knn_batch(tsg_ctx* ctx, const uint8_t* mask, uint32_t w, uint32_t h, uint32_t n_q, const uint32_t* q_flat,
+                  uint32_t k, int32_t* nbrs, uint32_t* counts);
+
+/* Random candidates (ms.rs:549-599) of stage items [item0, item0+n):
+ * stream Pcg32::seed_from_u64(stage_seed + i + 1). maps / sample masks are
+ * the filtered example maps. out: n*m (x, y, map) u32 triples. */
+int tsg_random_candidates(tsg_ctx* ctx, uint32_t n_maps, const tsg_image* maps, const tsg_image* sample_masks,
+                          uint64_t stage_seed, uint32_t item0, uint32_t n, uint32_t m, uint32_t* out);
+
+/* Frozen-state scoring (find_best_match / better_match, ms.rs:1184-1288)
+ * with host-supplied neighbour and candidate lists.
+ * state: color (w*h*4 u8), coord (w*h*3 u32), ids (w*h*2 u32) of the output.
+ * maps / guides: filtered example maps at the stage level (+ their guides,
+ * or NULL); target_guide: w x h level image or NULL.
+ * lut_color / lut_guide: 256-entry cost tables by |a-b| (the stage's
+ * (1-alpha)*cauchy and alpha*l2; lut_guide NULL when unguided).
+ * per item i: flat[i]; nbr_count[i] and nbrs[i*k + j] = (x, y);
+ * cand_count[i] and cands[(i*cmax + c)*6] = (x, y, map, kind 0=coherence
+ * 1=random, patch, patch map).
+ * outputs: full_cost[i*cmax + c] (no early exit), best[i], best_score[i]. */
+int tsg_score_batch(tsg_ctx* ctx, uint32_t w, uint32_t h, const uint8_t* color, const uint32_t* coord,
+                    const uint32_t* ids, uint32_t n_maps, const tsg_image* maps, const tsg_image* guides,
+                    const tsg_image* target_guide, const float* lut_color, const float* lut_guide, uint32_t k,
+                    uint32_t cmax, uint32_t n, const uint32_t* flat, const uint32_t* nbr_count, const int32_t* nbrs,
+                    const uint32_t* cand_count, const int32_t* cands, float* full_cost, uint32_t* best,
+                    float* best_score);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TSGPU_H */

@@ -1,0 +1,100 @@
+// Kernel argument blocks and launchers of the patch-search engine.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+
+namespace tsg {
+
+constexpr int kMaxK = 64;          // nearest_neighbors supported on the device
+constexpr uint32_t kBruteMax = 2048;  // resolved count up to which kNN scans the list
+constexpr int kTableRadius = 64;   // offset table radius of the ring scan
+
+// One Jacobi batch: items [item0, item1) of the current stage.
+struct BatchArgs {
+    uint32_t W, H, tiles_x, tiles_y;
+    const uint64_t* mask;    // resolved bits, one u64 per 8x8 tile
+    const uint4* state;      // per output pixel {colour, src xy, map ids, patch}
+    const uint32_t* tg;      // target guide at the stage level (null = unguided)
+    const MapDesc* maps;     // filtered example maps at the stage level
+    uint32_t n_maps;
+    const uint32_t* work;    // stage work list: flat pixel of item i
+    const uint32_t* rlist;   // resolved list (first n_res entries committed)
+    uint32_t n_res;
+    const uint2* rcands;     // random candidates, item-major, m per item
+    uint32_t rc_base;        // item index of rcands[0]
+    uint32_t m;
+    const int32_t* offtab;   // ring offsets (dx & 0xffff | dy << 16), canonical order
+    uint32_t offtab_n;
+    const float* lut;        // [0,256): colour cost by |a-b|; [256,512): guide cost
+    uint32_t item0, item1;
+    uint64_t stage_seed;
+    uint32_t k;
+    uint4* result;           // per batch item {src xy, map ids | fallback, patch, score bits}
+    float* item_score;       // per stage item (stats)
+    // test hooks: host-supplied neighbour / candidate lists (null in production)
+    const int32_t* ext_nbrs;      // [item][k] packed dx | dy << 16
+    const uint32_t* ext_nbr_count;
+    const uint4* ext_cands;       // [item][cmax] {xy, map | kind << 31, patch, 0}
+    const uint32_t* ext_cand_count;
+    uint32_t ext_cmax;
+    float* ext_full_cost;         // [item][cmax] full sums (no early exit)
+    int32_t* dbg_nbrs;            // [item][k] kNN result dump (test hook)
+    uint32_t* dbg_nbr_count;
+    uint32_t* dbg_best;           // [item] winning candidate index (test hook)
+    uint32_t knn_only;            // stop after the kNN dump (test hook)
+};
+
+struct RandArgs {
+    const MapDesc* maps;
+    uint32_t n_maps;
+    uint64_t stage_seed;
+    uint32_t item0, item1;
+    uint32_t m;
+    uint2* out;              // [item - item0][m]
+    uint32_t* overflow;      // set when a rejection loop hits its bound
+};
+
+struct CommitArgs {
+    uint32_t W, tiles_x;
+    const MapDesc* maps;
+    const uint32_t* work;
+    const uint4* result;
+    uint32_t item0, item1;
+    uint32_t first_new;      // items >= first_new are new pixels
+    uint4* state;
+    uint64_t* mask;
+    float* score;            // per pixel score of the first resolve
+};
+
+struct RecolorArgs {
+    const MapDesc* maps;
+    const uint32_t* rlist;
+    uint32_t n;
+    uint4* state;
+};
+
+// per-stage statistics: sum and count of non-negative item scores
+struct StatsArgs {
+    const float* item_score;
+    uint32_t n;
+    double* sum;
+    unsigned long long* count;
+};
+
+hipError_t configure_kernels();
+size_t resolve_lds_bytes(bool guided);
+hipError_t launch_resolve(const BatchArgs& a, bool guided, hipStream_t s);
+hipError_t launch_stats(const StatsArgs& a, hipStream_t s);
+hipError_t launch_rand(const RandArgs& a, hipStream_t s);
+hipError_t launch_commit(const CommitArgs& a, hipStream_t s);
+hipError_t launch_recolor(const RecolorArgs& a, hipStream_t s);
+// separable resampling pass: vertical (u8 -> f32) and horizontal (f32 -> u8)
+hipError_t launch_resample_v(const uint32_t* src, uint32_t w, uint32_t h, float4* dst, uint32_t nh,
+                             const uint32_t* left, const uint32_t* cnt, const float* ws, uint32_t stride,
+                             hipStream_t s);
+hipError_t launch_resample_h(const float4* src, uint32_t w, uint32_t h, uint32_t* dst, uint32_t nw,
+                             const uint32_t* left, const uint32_t* cnt, const float* ws, uint32_t stride,
+                             hipStream_t s);
+
+}  // namespace tsg

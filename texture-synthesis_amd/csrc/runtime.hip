@@ -1,0 +1,1197 @@
+// Host runtime of the patch-search engine: contexts, jobs (device-resident
+// pyramids and state), the backtracking stage schedule and the C ABI.
+//
+// Session::run (lib/src/session.rs:37-66) = random init (ms.rs:427-445) +
+// Generator::resolve (ms.rs:702-1051). Per stage the host derives the level,
+// stage seed, stage size, adaptive alpha and 256-entry cost tables
+// (ms.rs:786-858), picks the new pixels in the reference's order
+// (pick_random_unresolved, ms.rs:380-389), and enqueues Jacobi batches of
+// {k_resolve, k_commit} on one stream without waiting on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/tsgpu.h"
+#include "kernels.h"
+
+using namespace tsg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) {                                                                    \
+            int c_ = e_ == hipErrorOutOfMemory ? TSG_ERR_OUT_OF_MEMORY : TSG_ERR_DEVICE;           \
+            return fail(c_, std::string(#expr) + ": " + hipGetErrorString(e_));                    \
+        }                                                                                          \
+    } while (0)
+
+// Owning device allocation.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+    hipError_t reserve(size_t count) { return count <= n ? hipSuccess : alloc(count); }
+};
+
+// ---- image 0.23 separable resampling weights (imageops/sample.rs) ----
+struct Filter {
+    float (*kernel)(float);
+    float support;
+};
+float kern_triangle(float x) {
+    float a = std::fabs(x);
+    return a < 1.0f ? 1.0f - a : 0.0f;
+}
+float kern_catmullrom(float x) {
+    // bc_cubic_spline with B = 0, C = 0.5
+    const float b = 0.0f, c = 0.5f;
+    float a = std::fabs(x), k;
+    if (a < 1.0f)
+        k = (12.0f - 9.0f * b - 6.0f * c) * (a * a * a) + (-18.0f + 12.0f * b + 6.0f * c) * (a * a) + (6.0f - 2.0f * b);
+    else if (a < 2.0f)
+        k = (-b - 6.0f * c) * (a * a * a) + (6.0f * b + 30.0f * c) * (a * a) + (-12.0f * b - 48.0f * c) * a +
+            (8.0f * b + 24.0f * c);
+    else
+        k = 0.0f;
+    return k / 6.0f;
+}
+float kern_gaussian(float x) {
+    // gaussian(x, r = 0.5)
+    const float r = 0.5f;
+    float c = 1.0f / (std::sqrt(2.0f * 3.14159265358979323846f) * r);
+    return c * std::exp(-(x * x) / (2.0f * (r * r)));
+}
+
+struct Taps {
+    std::vector<uint32_t> left, cnt;
+    std::vector<float> ws;
+    uint32_t stride = 0;
+};
+
+Taps make_taps(uint32_t in_len, uint32_t out_len, const Filter& f) {
+    Taps t;
+    float ratio = (float)in_len / (float)out_len;
+    float sratio = ratio < 1.0f ? 1.0f : ratio;
+    float support = f.support * sratio;
+    std::vector<std::vector<float>> tmp(out_len);
+    t.left.resize(out_len);
+    t.cnt.resize(out_len);
+    for (uint32_t o = 0; o < out_len; ++o) {
+        float in = ((float)o + 0.5f) * ratio;
+        int64_t l = (int64_t)std::floor(in - support);
+        l = std::min<int64_t>(std::max<int64_t>(l, 0), (int64_t)in_len - 1);
+        int64_t r = (int64_t)std::ceil(in + support);
+        r = std::min<int64_t>(std::max<int64_t>(r, l + 1), (int64_t)in_len);
+        in = in - 0.5f;
+        float sum = 0.0f;
+        for (int64_t i = l; i < r; ++i) {
+            float w = f.kernel(((float)i - in) / sratio);
+            tmp[o].push_back(w);
+            sum += w;
+        }
+        for (float& w : tmp[o]) w /= sum;
+        t.left[o] = (uint32_t)l;
+        t.cnt[o] = (uint32_t)(r - l);
+        t.stride = std::max(t.stride, t.cnt[o]);
+    }
+    t.ws.assign((size_t)out_len * t.stride, 0.0f);
+    for (uint32_t o = 0; o < out_len; ++o)
+        for (uint32_t i = 0; i < t.cnt[o]; ++i) t.ws[(size_t)o * t.stride + i] = tmp[o][i];
+    return t;
+}
+
+struct DevTaps {
+    DBuf<uint32_t> left, cnt;
+    DBuf<float> ws;
+    uint32_t stride = 0;
+    hipError_t upload(const Taps& t, hipStream_t s) {
+        hipError_t e;
+        if ((e = left.alloc(t.left.size())) != hipSuccess) return e;
+        if ((e = cnt.alloc(t.cnt.size())) != hipSuccess) return e;
+        if ((e = ws.alloc(t.ws.size())) != hipSuccess) return e;
+        stride = t.stride;
+        if ((e = hipMemcpyAsync(left.p, t.left.data(), t.left.size() * 4, hipMemcpyHostToDevice, s)) != hipSuccess)
+            return e;
+        if ((e = hipMemcpyAsync(cnt.p, t.cnt.data(), t.cnt.size() * 4, hipMemcpyHostToDevice, s)) != hipSuccess)
+            return e;
+        return hipMemcpyAsync(ws.p, t.ws.data(), t.ws.size() * 4, hipMemcpyHostToDevice, s);
+    }
+};
+
+// imageops::resize(img, nw, nh, filter) on the device: vertical then
+// horizontal pass (f32 in between, rounded u8 out).
+int device_resize(const uint32_t* src, uint32_t w, uint32_t h, uint32_t* dst, uint32_t nw, uint32_t nh,
+                  const Filter& f, hipStream_t s) {
+    Taps tv = make_taps(h, nh, f), th = make_taps(w, nw, f);
+    DevTaps dv, dh;
+    DBuf<float4> tmp;
+    HIP_TRY(dv.upload(tv, s));
+    HIP_TRY(dh.upload(th, s));
+    HIP_TRY(tmp.alloc((size_t)w * nh));
+    HIP_TRY(launch_resample_v(src, w, h, tmp.p, nh, dv.left.p, dv.cnt.p, dv.ws.p, dv.stride, s));
+    HIP_TRY(launch_resample_h(tmp.p, w, nh, dst, nw, dh.left.p, dh.cnt.p, dh.ws.p, dh.stride, s));
+    HIP_TRY(hipStreamSynchronize(s));  // taps and tmp die here
+    return TSG_OK;
+}
+
+// ImagePyramid::build_gaussian (img_pyramid.rs:20-37): `levels` full-size
+// levels at out, coarsest first; the last is the input itself.
+int device_pyramid(const uint8_t* host_rgba, uint32_t w, uint32_t h, uint32_t levels, uint32_t* out,
+                   hipStream_t s) {
+    const Filter g{kern_gaussian, 3.0f};
+    size_t npx = (size_t)w * h;
+    uint32_t* last = out + (size_t)(levels - 1) * npx;
+    HIP_TRY(hipMemcpyAsync(last, host_rgba, npx * 4, hipMemcpyHostToDevice, s));
+    DBuf<uint32_t> small;
+    for (uint32_t lv = 0; lv + 1 < levels; ++lv) {
+        uint32_t i = levels - 1 - lv;
+        uint32_t pw = w >> i, ph = h >> i;
+        if (pw == 0 || ph == 0) return fail(TSG_ERR_INVALID_ARGUMENT, "image too small for the pyramid depth");
+        HIP_TRY(small.reserve((size_t)pw * ph));
+        int rc = device_resize(last, w, h, small.p, pw, ph, g, s);
+        if (rc) return rc;
+        rc = device_resize(small.p, pw, ph, out + (size_t)lv * npx, w, h, g, s);
+        if (rc) return rc;
+    }
+    return TSG_OK;
+}
+
+const std::vector<int32_t>& offset_table() {
+    static std::vector<int32_t> t = [] {
+        struct O {
+            int dx, dy;
+        };
+        std::vector<O> v;
+        const int R = kTableRadius;
+        for (int dy = -R; dy <= R; ++dy)
+            for (int dx = -R; dx <= R; ++dx)
+                if (dx * dx + dy * dy <= R * R) v.push_back({dx, dy});
+        std::sort(v.begin(), v.end(), [](const O& a, const O& b) {
+            int da = a.dx * a.dx + a.dy * a.dy, db = b.dx * b.dx + b.dy * b.dy;
+            if (da != db) return da < db;
+            if (a.dy != b.dy) return a.dy < b.dy;
+            return a.dx < b.dx;
+        });
+        std::vector<int32_t> out;
+        for (const O& o : v) out.push_back((int32_t)(((uint32_t)o.dx & 0xffffu) | ((uint32_t)o.dy << 16)));
+        return out;
+    }();
+    return t;
+}
+
+inline uint32_t pack_px(const uint8_t* p) {
+    return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+}
+
+// metric_cauchy / metric_l2 (ms.rs:1109-1120) for |a-b| = d; the reference's
+// 256x256 tables depend only on |a-b| (exact in f32: the difference of two
+// u8 values is exact and its square removes the sign).
+void stage_luts(float cauchy, bool guided, float alpha_s, float* out512) {
+    float sig2 = cauchy * cauchy;
+    for (int d = 0; d < 256; ++d) {
+        float x = ((float)d - 0.0f) / 255.0f;
+        float x2 = x * x;
+        float c = std::log1p(x2 / sig2);
+        float l2 = x * x;
+        out512[d] = guided ? (1.0f - alpha_s) * c : c;
+        out512[256 + d] = guided ? alpha_s * l2 : 0.0f;
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------
+struct tsg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DBuf<int32_t> offtab;
+    uint32_t offtab_n = 0;
+    DBuf<float> dummy_lut;
+    std::mutex mu;
+};
+
+struct tsg_job {
+    tsg_ctx* ctx = nullptr;
+    tsg_params prm{};
+    tsg_schedule sched{};
+    uint32_t W = 0, H = 0, tiles_x = 0, tiles_y = 0;
+    size_t N = 0;
+    uint32_t n_ex = 0, levels = 1;
+    std::vector<uint32_t> filt;
+    std::vector<int> kind;
+    bool guided = false;
+    std::vector<std::pair<uint32_t, uint32_t>> ex_dims, g_dims;
+    std::vector<std::unique_ptr<DBuf<uint32_t>>> ex, exg;  // [example] levels x w x h
+    std::vector<std::unique_ptr<DBuf<uint8_t>>> smask;    // [example] (null = All)
+    DBuf<uint32_t> tg;                                    // levels x W x H
+    DBuf<MapDesc> maps;                                   // [level][filtered map]
+    std::vector<std::vector<uint32_t>> h_finest;          // finest example pixels
+    // initial state inputs
+    bool inpaint = false;
+    std::vector<uint8_t> inpaint_mask_r;
+    uint32_t inpaint_index = 0;
+    // device state
+    DBuf<uint4> state;
+    DBuf<uint64_t> mask;
+    DBuf<float> score;
+    DBuf<uint32_t> rlist;
+    // scratch
+    DBuf<uint4> result;
+    DBuf<uint2> rc;
+    DBuf<float> item_score;
+    DBuf<float> lut;
+    DBuf<uint32_t> overflow;
+    DBuf<double> st_sum;
+    DBuf<unsigned long long> st_cnt;
+    // host bookkeeping
+    std::vector<uint32_t> unresolved;
+    size_t n_res = 0, locked = 0, N0 = 0;
+    std::vector<uint64_t> st_items;
+    std::vector<std::vector<uint32_t>> picks_keep;
+    std::vector<std::vector<float>> lut_keep;
+    bool ran = false;
+    bool state_ready = false;
+    double run_ms = 0.0;
+    uint64_t n_batches = 0;
+    // optional per-kernel device timing (events on the job's stream)
+    bool profile = false;
+    struct Timed {
+        int kind;
+        uint64_t items;
+        hipEvent_t a, b;
+    };
+    std::vector<Timed> timed;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_next = 0;
+    double kern_ms[5] = {0, 0, 0, 0, 0};
+    uint64_t kern_items[5] = {0, 0, 0, 0, 0};
+    uint64_t kern_launches[5] = {0, 0, 0, 0, 0};
+    ~tsg_job() {
+        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+    }
+};
+
+namespace {
+
+uint64_t stage_size(const tsg_job* j, int s) {
+    float v = std::pow(j->prm.backtrack_percent, (float)s) * (float)j->N0;
+    if (!(v > 0.0f)) return 0;
+    if (v >= 18446744073709551615.0f) return UINT64_MAX;
+    return (uint64_t)v;
+}
+
+// pick_random_unresolved for items [i0, i0 + n): index draws in parallel
+// (each depends only on its seed and the list length at that step), then
+// the swap_remove chain in order.
+std::vector<uint32_t> pick_new(tsg_job* j, uint64_t stage_seed, uint64_t i0, uint64_t n) {
+    size_t L0 = j->unresolved.size();
+    n = std::min<uint64_t>(n, L0);
+    std::vector<uint64_t> idx(n);
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 4096) nt = 1;
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t) {
+        th.emplace_back([&, t] {
+            for (uint64_t q = t; q < n; q += nt) {
+                Pcg32 g = Pcg32::seed_from_u64(stage_seed + i0 + q);
+                idx[q] = range_u64(g, 0, L0 - q);
+            }
+        });
+    }
+    for (auto& x : th) x.join();
+    std::vector<uint32_t> out(n);
+    for (uint64_t q = 0; q < n; ++q) {
+        uint64_t k = idx[q];
+        out[q] = j->unresolved[k];
+        j->unresolved[k] = j->unresolved.back();
+        j->unresolved.pop_back();
+    }
+    return out;
+}
+
+// resolve_at_random source (ms.rs:447-475): three identically seeded streams.
+void random_source(uint64_t seed, const std::vector<std::pair<uint32_t, uint32_t>>& dims, uint32_t& m, uint32_t& x,
+                   uint32_t& y) {
+    Pcg32 a = Pcg32::seed_from_u64(seed);
+    m = (uint32_t)range_u64(a, 0, dims.size());
+    Pcg32 b = Pcg32::seed_from_u64(seed);
+    x = range_u32(b, 0, dims[m].first);
+    Pcg32 c = Pcg32::seed_from_u64(seed);
+    y = range_u32(c, 0, dims[m].second);
+}
+
+int check_range(float v, float lo, float hi, const char* name) {
+    if (v < lo || v > hi) {
+        return fail(TSG_ERR_INVALID_RANGE, std::string("parameter '") + name + "' - value '" + std::to_string(v) +
+                                               "' is outside the range of " + std::to_string(lo) + "-" +
+                                               std::to_string(hi));
+    }
+    return TSG_OK;
+}
+
+// Initial state: Generator::new / new_from_inpaint (ms.rs:220-293) and the
+// random init of Session::run (session.rs:42-52).
+int init_state(tsg_job* j) {
+    const size_t N = j->N;
+    std::vector<uint4> st(N, make_uint4(0, 0, 0, 0));
+    std::vector<uint64_t> mask((size_t)j->tiles_x * j->tiles_y, 0);
+    std::vector<uint32_t> rl;
+    rl.reserve(N);
+    auto set_bit = [&](uint32_t f) {
+        uint32_t x = f % j->W, y = f / j->W;
+        mask[(y >> 3) * j->tiles_x + (x >> 3)] |= 1ull << (((y & 7u) << 3) | (x & 7u));
+    };
+    j->unresolved.clear();
+    if (j->inpaint) {
+        const std::vector<uint32_t>& col = j->h_finest[j->inpaint_index];
+        for (size_t i = 0; i < N; ++i) {
+            st[i].x = col[i];
+            if (j->inpaint_mask_r[i] < 255) {
+                j->unresolved.push_back((uint32_t)i);
+            } else {
+                uint32_t x = (uint32_t)(i % j->W), y = (uint32_t)(i / j->W);
+                st[i] = make_uint4(col[i], x | (y << 16), j->inpaint_index, 0u);
+                set_bit((uint32_t)i);
+                rl.push_back((uint32_t)i);
+            }
+        }
+    } else {
+        j->unresolved.resize(N);
+        for (size_t i = 0; i < N; ++i) j->unresolved[i] = (uint32_t)i;
+    }
+    j->locked = rl.size();
+    if (j->prm.random_init != UINT64_MAX) {
+        for (uint64_t i = 0; i < j->prm.random_init; ++i) {
+            if (j->unresolved.empty()) continue;
+            Pcg32 g = Pcg32::seed_from_u64(j->prm.seed + i);
+            uint64_t k = range_u64(g, 0, j->unresolved.size());
+            uint32_t f = j->unresolved[k];
+            j->unresolved[k] = j->unresolved.back();
+            j->unresolved.pop_back();
+            uint32_t m, x, y;
+            random_source(j->prm.seed + i + (uint64_t)f, j->ex_dims, m, x, y);
+            uint32_t c = j->h_finest[m][(size_t)y * j->ex_dims[m].first + x];
+            st[f] = make_uint4(c, x | (y << 16), m | (m << 16), f);
+            set_bit(f);
+            rl.push_back(f);
+        }
+        j->locked += j->prm.random_init;
+        if (j->locked > rl.size()) return fail(TSG_ERR_INVALID_ARGUMENT, "random_init exceeds the pixels to resolve");
+    }
+    j->n_res = rl.size();
+    j->N0 = j->unresolved.size();
+    hipStream_t s = j->ctx->stream;
+    HIP_TRY(hipMemcpyAsync(j->state.p, st.data(), N * sizeof(uint4), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(j->mask.p, mask.data(), mask.size() * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemsetAsync(j->score.p, 0, N * sizeof(float), s));
+    if (!rl.empty()) HIP_TRY(hipMemcpyAsync(j->rlist.p, rl.data(), rl.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return TSG_OK;
+}
+
+uint64_t batch_end(const tsg_job* j, uint64_t i, uint64_t redo, uint64_t S_eff) {
+    uint64_t redo_eff = std::min<uint64_t>(redo, S_eff);
+    uint64_t seg_end = i < redo_eff ? redo_eff : S_eff;
+    uint64_t b;
+    if (j->sched.px_per_batch > 0) {
+        b = j->sched.px_per_batch;
+    } else {
+        uint64_t base = j->n_res > j->locked ? j->n_res - j->locked : 0;
+        b = base >> (i < redo_eff ? j->sched.redo_shift : j->sched.new_shift);
+        if (b < 1) b = 1;
+        if (j->sched.max_batch > 0 && b > j->sched.max_batch) b = j->sched.max_batch;
+    }
+    return std::min(i + b, seg_end);
+}
+
+enum { K_RESOLVE = 0, K_COMMIT = 1, K_RAND = 2, K_RECOLOR = 3, K_STATS = 4 };
+
+hipError_t next_event(tsg_job* j, hipEvent_t* e) {
+    if (j->ev_next == j->ev_pool.size()) {
+        hipEvent_t n;
+        hipError_t r = hipEventCreate(&n);
+        if (r != hipSuccess) return r;
+        j->ev_pool.push_back(n);
+    }
+    *e = j->ev_pool[j->ev_next++];
+    return hipSuccess;
+}
+
+// Runs `launch` on the job stream, bracketed by events when profiling.
+template <class F>
+hipError_t timed_launch(tsg_job* j, int kind, uint64_t items, F&& launch) {
+    if (!j->profile) return launch();
+    tsg_job::Timed t{kind, items, nullptr, nullptr};
+    hipError_t e;
+    if ((e = next_event(j, &t.a)) != hipSuccess) return e;
+    if ((e = next_event(j, &t.b)) != hipSuccess) return e;
+    if ((e = hipEventRecord(t.a, j->ctx->stream)) != hipSuccess) return e;
+    if ((e = launch()) != hipSuccess) return e;
+    if ((e = hipEventRecord(t.b, j->ctx->stream)) != hipSuccess) return e;
+    j->timed.push_back(t);
+    return hipSuccess;
+}
+
+int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
+    hipStream_t s = j->ctx->stream;
+    if (!j->state_ready) return fail(TSG_ERR_INVALID_ARGUMENT, "job state was not initialised");
+    j->timed.clear();
+    j->ev_next = 0;
+    for (int q = 0; q < 5; ++q) j->kern_ms[q] = 0.0, j->kern_items[q] = 0, j->kern_launches[q] = 0;
+    const int P = (int)j->prm.backtrack_stages;
+    const uint32_t nf = (uint32_t)j->filt.size();
+    const uint32_t m = (uint32_t)j->prm.random_sample_locations;
+    j->st_items.assign(P + 1, 0);
+    HIP_TRY(hipMemsetAsync(j->st_sum.p, 0, (P + 1) * sizeof(double), s));
+    HIP_TRY(hipMemsetAsync(j->st_cnt.p, 0, (P + 1) * sizeof(unsigned long long), s));
+    HIP_TRY(hipMemsetAsync(j->overflow.p, 0, 4, s));
+    j->picks_keep.assign(P + 1, {});
+    j->lut_keep.assign(P + 1, std::vector<float>(512));
+    j->n_batches = 0;
+
+    uint64_t overall_total = 0;
+    for (int t = 0; t <= P; ++t) overall_total += stage_size(j, P - t);
+    uint64_t overall_cur = 0;
+    int last_pct = 0;
+    std::vector<uint8_t> snap;
+
+    hipEvent_t ev0, ev1;
+    HIP_TRY(hipEventCreate(&ev0));
+    HIP_TRY(hipEventCreate(&ev1));
+    HIP_TRY(hipEventRecord(ev0, s));
+
+    for (int t = 0; t <= P; ++t) {
+        const int ps = P - t;
+        const uint32_t level = t == 0 ? 0u : (uint32_t)std::min(t, P - 1);
+        const MapDesc* maps = j->maps.p + (size_t)level * nf;
+        if (level > 0) {
+            RecolorArgs ra{maps, j->rlist.p, (uint32_t)j->n_res, j->state.p};
+            HIP_TRY(timed_launch(j, K_RECOLOR, j->n_res, [&] { return launch_recolor(ra, s); }));
+        }
+        const uint64_t pss = (uint64_t)Pcg32::seed_from_u64(j->prm.seed + (uint64_t)ps).next_u32();
+        const uint64_t S = stage_size(j, ps);
+        const uint64_t redo = j->n_res - j->locked;
+        float alpha_s = 0.0f;
+        if (j->guided && ps > 0) {
+            float b = j->prm.guide_alpha * (1.0f - ((float)j->n_res / (float)j->N0));
+            alpha_s = (b * b) * b;
+        }
+        stage_luts(j->prm.cauchy_dispersion, j->guided, alpha_s, j->lut_keep[t].data());
+        HIP_TRY(hipMemcpyAsync(j->lut.p, j->lut_keep[t].data(), 512 * 4, hipMemcpyHostToDevice, s));
+        uint64_t S_eff = S;
+        if (S > redo) {
+            j->picks_keep[t] = pick_new(j, pss, redo, S - redo);
+            S_eff = redo + j->picks_keep[t].size();
+            if (!j->picks_keep[t].empty())
+                HIP_TRY(hipMemcpyAsync(j->rlist.p + j->locked + redo, j->picks_keep[t].data(),
+                                       j->picks_keep[t].size() * 4, hipMemcpyHostToDevice, s));
+        }
+        const uint64_t redo_eff = std::min(redo, S_eff);
+        const uint32_t* work = j->rlist.p + j->locked;
+        uint64_t rc_lo = 0, rc_hi = 0;
+        for (uint64_t i = 0; i < S_eff;) {
+            uint64_t e = batch_end(j, i, redo, S_eff);
+            if (e > rc_hi) {
+                uint64_t cap_items = std::max<uint64_t>(e - i, (uint64_t)(32u << 20) / (8ull * std::max(1u, m)));
+                rc_lo = i;
+                rc_hi = std::min<uint64_t>(S_eff, i + cap_items);
+                HIP_TRY(j->rc.reserve((rc_hi - rc_lo) * m));
+                RandArgs ra{maps, nf, pss, (uint32_t)rc_lo, (uint32_t)rc_hi, m, j->rc.p, j->overflow.p};
+                HIP_TRY(timed_launch(j, K_RAND, rc_hi - rc_lo, [&] { return launch_rand(ra, s); }));
+            }
+            HIP_TRY(j->result.reserve(e - i));
+            BatchArgs a{};
+            a.W = j->W;
+            a.H = j->H;
+            a.tiles_x = j->tiles_x;
+            a.tiles_y = j->tiles_y;
+            a.mask = j->mask.p;
+            a.state = j->state.p;
+            a.tg = j->guided ? j->tg.p + (size_t)level * j->N : nullptr;
+            a.maps = maps;
+            a.n_maps = nf;
+            a.work = work;
+            a.rlist = j->rlist.p;
+            a.n_res = (uint32_t)j->n_res;
+            a.rcands = j->rc.p;
+            a.rc_base = (uint32_t)rc_lo;
+            a.m = m;
+            a.offtab = j->ctx->offtab.p;
+            a.offtab_n = j->ctx->offtab_n;
+            a.lut = j->lut.p;
+            a.item0 = (uint32_t)i;
+            a.item1 = (uint32_t)e;
+            a.stage_seed = pss;
+            a.k = j->prm.nearest_neighbors;
+            a.result = j->result.p;
+            a.item_score = j->item_score.p;
+            HIP_TRY(timed_launch(j, K_RESOLVE, e - i, [&] { return launch_resolve(a, j->guided, s); }));
+            CommitArgs ca{j->W, j->tiles_x, maps, work, j->result.p, (uint32_t)i, (uint32_t)e, (uint32_t)redo_eff,
+                          j->state.p, j->mask.p, j->score.p};
+            HIP_TRY(timed_launch(j, K_COMMIT, e - i, [&] { return launch_commit(ca, s); }));
+            if (e > redo_eff) j->n_res += e - std::max(i, redo_eff);
+            j->n_batches += 1;
+            i = e;
+            if (cb) {
+                int pct = (int)std::lround((double)(overall_cur + i) / (double)std::max<uint64_t>(overall_total, 1) *
+                                           100.0);
+                if (pct != last_pct) {
+                    last_pct = pct;
+                    snap.resize(j->N * 4);
+                    HIP_TRY(hipStreamSynchronize(s));
+                    std::vector<uint4> stv(j->N);
+                    HIP_TRY(hipMemcpy(stv.data(), j->state.p, j->N * sizeof(uint4), hipMemcpyDeviceToHost));
+                    for (size_t q = 0; q < j->N; ++q) memcpy(&snap[q * 4], &stv[q].x, 4);
+                    cb(user, snap.data(), j->W, j->H, overall_cur + i, overall_total, i, S);
+                }
+            }
+        }
+        StatsArgs sa{j->item_score.p, (uint32_t)S_eff, j->st_sum.p + t, j->st_cnt.p + t};
+        HIP_TRY(timed_launch(j, K_STATS, S_eff, [&] { return launch_stats(sa, s); }));
+        j->st_items[t] = S_eff;
+        overall_cur += S;
+    }
+    HIP_TRY(hipEventRecord(ev1, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    j->run_ms = ms;
+    for (const tsg_job::Timed& t : j->timed) {
+        float km = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&km, t.a, t.b));
+        j->kern_ms[t.kind] += km;
+        j->kern_items[t.kind] += t.items;
+        j->kern_launches[t.kind] += 1;
+    }
+    uint32_t ovf = 0;
+    HIP_TRY(hipMemcpy(&ovf, j->overflow.p, 4, hipMemcpyDeviceToHost));
+    if (ovf) return fail(TSG_ERR_SAMPLING, "a random-candidate rejection loop hit its bound (sample mask too sparse)");
+    j->ran = true;
+    return TSG_OK;
+}
+
+// Upload one image as RGBA8 packed u32.
+std::vector<uint32_t> packed(const tsg_image& im) {
+    std::vector<uint32_t> v((size_t)im.w * im.h);
+    for (size_t i = 0; i < v.size(); ++i) v[i] = pack_px(im.rgba + i * 4);
+    return v;
+}
+
+bool valid_image(const tsg_image* im) { return im && im->rgba && im->w > 0 && im->h > 0; }
+
+}  // namespace
+
+// ------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------
+extern "C" {
+
+void tsg_params_default(tsg_params* p) {
+    p->nearest_neighbors = 50;
+    p->random_sample_locations = 50;
+    p->cauchy_dispersion = 1.0f;
+    p->backtrack_percent = 0.5f;
+    p->backtrack_stages = 5;
+    p->guide_alpha = 0.8f;
+    p->seed = 0;
+    p->tiling_mode = 0;
+    p->random_init = UINT64_MAX;
+    p->max_thread_count = 1;
+}
+
+void tsg_schedule_default(tsg_schedule* s) {
+    s->px_per_batch = 0;
+    s->redo_shift = 5;
+    s->new_shift = 6;
+    s->max_batch = 0;
+}
+
+const char* tsg_last_error(void) { return g_err.c_str(); }
+const char* tsg_version(void) { return "tsgpu 0.1 gfx950"; }
+
+int tsg_ctx_create(int device, tsg_ctx** out) {
+    if (!out) return fail(TSG_ERR_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(TSG_ERR_INVALID_ARGUMENT, "no such device");
+    HIP_TRY(hipSetDevice(device));
+    std::unique_ptr<tsg_ctx> c(new tsg_ctx());
+    c->device = device;
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIP_TRY(configure_kernels());
+    const std::vector<int32_t>& tab = offset_table();
+    HIP_TRY(c->offtab.alloc(tab.size()));
+    HIP_TRY(hipMemcpy(c->offtab.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice));
+    c->offtab_n = (uint32_t)tab.size();
+    HIP_TRY(c->dummy_lut.alloc(512));
+    HIP_TRY(hipMemset(c->dummy_lut.p, 0, 512 * 4));
+    *out = c.release();
+    return TSG_OK;
+}
+
+void tsg_ctx_destroy(tsg_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->offtab.release();
+    ctx->dummy_lut.release();
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int tsg_job_create(tsg_ctx* ctx, const tsg_params* prm, uint32_t n_examples, const tsg_image* examples,
+                   const int* sample_kind, const tsg_image* sample_masks, const tsg_image* example_guides,
+                   const tsg_image* target_guide, const tsg_image* inpaint_mask, uint32_t inpaint_index,
+                   uint32_t out_w, uint32_t out_h, tsg_job** out) {
+    if (!ctx || !prm || !out) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    // check_parameters_validity (session.rs:450-499)
+    int rc;
+    if ((rc = check_range(prm->cauchy_dispersion, 0.0f, 1.0f, "cauchy-dispersion"))) return rc;
+    if ((rc = check_range(prm->backtrack_percent, 0.0f, 1.0f, "backtrack-percent"))) return rc;
+    if ((rc = check_range(prm->guide_alpha, 0.0f, 1.0f, "guide-alpha"))) return rc;
+    if (prm->max_thread_count == 0) return check_range(0.0f, 1.0f, 1024.0f, "max-thread-count");
+    if (prm->random_sample_locations == 0) return check_range(0.0f, 1.0f, 1024.0f, "m-rand");
+    // check_images_validity (session.rs:501-524)
+    if (n_examples == 0 || !examples) return fail(TSG_ERR_NO_EXAMPLES, "at least 1 example must be available as a sampling source");
+    uint32_t usable = 0;
+    for (uint32_t i = 0; i < n_examples; ++i)
+        if (!sample_kind || sample_kind[i] != TSG_SAMPLE_IGNORE) ++usable;
+    if (usable == 0) return fail(TSG_ERR_NO_EXAMPLES, "at least 1 example must be available as a sampling source");
+    if (example_guides && !target_guide)
+        return fail(TSG_ERR_UNSUPPORTED, "example guides without a target guide are not used by the reference");
+    // device-path limits
+    if (prm->nearest_neighbors == 0 || prm->nearest_neighbors > (uint32_t)kMaxK)
+        return fail(TSG_ERR_UNSUPPORTED, "nearest_neighbors must be in 1..64 on the device");
+    if (prm->random_sample_locations > 4096) return fail(TSG_ERR_UNSUPPORTED, "random_sample_locations above 4096");
+    if (prm->tiling_mode) return fail(TSG_ERR_UNSUPPORTED, "tiling mode is not on the device path yet");
+    if (target_guide && !example_guides)
+        return fail(TSG_ERR_UNSUPPORTED, "style transfer (target guide without example guides) needs host guide prep");
+    if (out_w == 0 || out_h == 0 || out_w > 16384 || out_h > 16384)
+        return fail(TSG_ERR_INVALID_ARGUMENT, "output size must be 1..16384 per side");
+    for (uint32_t i = 0; i < n_examples; ++i) {
+        if (!valid_image(&examples[i])) return fail(TSG_ERR_INVALID_ARGUMENT, "invalid example image");
+        if (examples[i].w > 65535 || examples[i].h > 65535)
+            return fail(TSG_ERR_UNSUPPORTED, "example side above 65535");
+        int kd = sample_kind ? sample_kind[i] : TSG_SAMPLE_ALL;
+        if (kd == TSG_SAMPLE_IMAGE) {
+            if (!sample_masks || !valid_image(&sample_masks[i]))
+                return fail(TSG_ERR_INVALID_ARGUMENT, "missing sample mask");
+            if (sample_masks[i].w < examples[i].w || sample_masks[i].h < examples[i].h)
+                return fail(TSG_ERR_SIZE_MISMATCH, "sample mask smaller than its example");
+        }
+        if (example_guides && !valid_image(&example_guides[i]))
+            return fail(TSG_ERR_INVALID_ARGUMENT, "invalid example guide");
+    }
+    if (target_guide && (!valid_image(target_guide) || target_guide->w != out_w || target_guide->h != out_h))
+        return fail(TSG_ERR_SIZE_MISMATCH, "target guide must have the output size");
+    if (inpaint_mask) {
+        if (!valid_image(inpaint_mask) || inpaint_mask->w != out_w || inpaint_mask->h != out_h)
+            return fail(TSG_ERR_SIZE_MISMATCH, "inpaint mask must have the output size");
+        if (inpaint_index >= n_examples) return fail(TSG_ERR_INVALID_ARGUMENT, "inpaint index out of range");
+        if (examples[inpaint_index].w != out_w || examples[inpaint_index].h != out_h)
+            return fail(TSG_ERR_SIZE_MISMATCH, "the inpainted example must have the output size");
+    }
+    if (prm->random_init != UINT64_MAX && usable != n_examples)
+        return fail(TSG_ERR_UNSUPPORTED, "random_init with ignored examples mixes map numbering (reference quirk)");
+
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::unique_ptr<tsg_job> j(new tsg_job());
+    j->ctx = ctx;
+    j->prm = *prm;
+    tsg_schedule_default(&j->sched);
+    j->W = out_w;
+    j->H = out_h;
+    j->N = (size_t)out_w * out_h;
+    j->tiles_x = (out_w + 7) / 8;
+    j->tiles_y = (out_h + 7) / 8;
+    j->n_ex = n_examples;
+    j->levels = prm->backtrack_stages == 0 ? 1 : prm->backtrack_stages;
+    j->guided = target_guide != nullptr;
+    for (uint32_t i = 0; i < n_examples; ++i) {
+        int kd = sample_kind ? sample_kind[i] : TSG_SAMPLE_ALL;
+        j->kind.push_back(kd);
+        if (kd != TSG_SAMPLE_IGNORE) j->filt.push_back(i);
+    }
+    if (inpaint_mask && inpaint_index >= j->filt.size())
+        return fail(TSG_ERR_UNSUPPORTED, "inpaint example index beyond the sampled examples (the reference panics)");
+    hipStream_t s = ctx->stream;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    for (uint32_t i = 0; i < n_examples; ++i) {
+        const tsg_image& im = examples[i];
+        j->ex_dims.push_back({im.w, im.h});
+        j->h_finest.push_back(packed(im));
+        std::unique_ptr<DBuf<uint32_t>> b(new DBuf<uint32_t>());
+        HIP_TRY(b->alloc((size_t)j->levels * im.w * im.h));
+        if ((rc = device_pyramid(im.rgba, im.w, im.h, j->levels, b->p, s))) return rc;
+        j->ex.push_back(std::move(b));
+        std::unique_ptr<DBuf<uint8_t>> mk(new DBuf<uint8_t>());
+        if (j->kind[i] == TSG_SAMPLE_IMAGE) {
+            const tsg_image& sm = sample_masks[i];
+            std::vector<uint8_t> v((size_t)im.w * im.h);
+            size_t valid = 0;
+            for (uint32_t y = 0; y < im.h; ++y)
+                for (uint32_t x = 0; x < im.w; ++x) {
+                    uint8_t r = sm.rgba[((size_t)y * sm.w + x) * 4];
+                    v[(size_t)y * im.w + x] = r != 0;
+                    valid += r != 0;
+                }
+            if (valid == 0)
+                return fail(TSG_ERR_SAMPLING, "sample mask admits no pixel (the reference loops forever)");
+            HIP_TRY(mk->alloc(v.size()));
+            HIP_TRY(hipMemcpy(mk->p, v.data(), v.size(), hipMemcpyHostToDevice));
+        }
+        j->smask.push_back(std::move(mk));
+        if (example_guides) {
+            const tsg_image& g = example_guides[i];
+            j->g_dims.push_back({g.w, g.h});
+            std::unique_ptr<DBuf<uint32_t>> gb(new DBuf<uint32_t>());
+            HIP_TRY(gb->alloc((size_t)j->levels * g.w * g.h));
+            if ((rc = device_pyramid(g.rgba, g.w, g.h, j->levels, gb->p, s))) return rc;
+            j->exg.push_back(std::move(gb));
+        }
+    }
+    if (target_guide) {
+        HIP_TRY(j->tg.alloc((size_t)j->levels * j->N));
+        if ((rc = device_pyramid(target_guide->rgba, out_w, out_h, j->levels, j->tg.p, s))) return rc;
+    }
+    // map descriptors per level; the guide of filtered map m is the m-th
+    // entry of the unfiltered guide list (get_single_guide_level, ms.rs:1566-1573)
+    const uint32_t nf = (uint32_t)j->filt.size();
+    std::vector<MapDesc> md((size_t)j->levels * nf);
+    for (uint32_t lv = 0; lv < j->levels; ++lv)
+        for (uint32_t mi = 0; mi < nf; ++mi) {
+            uint32_t e = j->filt[mi];
+            MapDesc d{};
+            d.w = j->ex_dims[e].first;
+            d.h = j->ex_dims[e].second;
+            d.px = j->ex[e]->p + (size_t)lv * d.w * d.h;
+            d.smask = j->smask[e]->p;
+            if (j->guided) {
+                d.gw = j->g_dims[mi].first;
+                d.gh = j->g_dims[mi].second;
+                d.gpx = j->exg[mi]->p + (size_t)lv * d.gw * d.gh;
+            }
+            md[(size_t)lv * nf + mi] = d;
+        }
+    HIP_TRY(j->maps.alloc(md.size()));
+    HIP_TRY(hipMemcpy(j->maps.p, md.data(), md.size() * sizeof(MapDesc), hipMemcpyHostToDevice));
+    if (inpaint_mask) {
+        j->inpaint = true;
+        j->inpaint_index = inpaint_index;
+        j->inpaint_mask_r.resize(j->N);
+        for (size_t i = 0; i < j->N; ++i) j->inpaint_mask_r[i] = inpaint_mask->rgba[i * 4];
+    }
+    const uint32_t P = prm->backtrack_stages;
+    HIP_TRY(j->state.alloc(j->N));
+    HIP_TRY(j->mask.alloc((size_t)j->tiles_x * j->tiles_y));
+    HIP_TRY(j->score.alloc(j->N));
+    HIP_TRY(j->rlist.alloc(j->N));
+    HIP_TRY(j->item_score.alloc(j->N));
+    HIP_TRY(j->lut.alloc(512));
+    HIP_TRY(j->overflow.alloc(1));
+    HIP_TRY(j->st_sum.alloc(P + 1));
+    HIP_TRY(j->st_cnt.alloc(P + 1));
+    // Generator::new / new_from_inpaint and the random init are content-
+    // independent of the stages: upload the initial state now
+    if ((rc = init_state(j.get()))) return rc;
+    j->state_ready = true;
+    HIP_TRY(hipStreamSynchronize(s));
+    *out = j.release();
+    return TSG_OK;
+}
+
+int tsg_job_set_schedule(tsg_job* job, const tsg_schedule* s) {
+    if (!job || !s) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    if (s->redo_shift > 31 || s->new_shift > 31) return fail(TSG_ERR_INVALID_ARGUMENT, "shift above 31");
+    job->sched = *s;
+    return TSG_OK;
+}
+
+int tsg_job_run(tsg_job* job, tsg_progress_fn progress, void* user) {
+    if (!job) return fail(TSG_ERR_INVALID_ARGUMENT, "null job");
+    if (job->ran) return fail(TSG_ERR_INVALID_ARGUMENT, "a job runs once (Session::run consumes the session)");
+    HIP_TRY(hipSetDevice(job->ctx->device));
+    std::lock_guard<std::mutex> lk(job->ctx->mu);
+    return run_job(job, progress, user);
+}
+
+int tsg_job_download(tsg_job* job, uint8_t* rgba, uint32_t* coord, uint32_t* ids, uint32_t* resolved_flat,
+                     float* resolved_score) {
+    if (!job) return fail(TSG_ERR_INVALID_ARGUMENT, "null job");
+    HIP_TRY(hipSetDevice(job->ctx->device));
+    std::vector<uint4> st(job->N);
+    HIP_TRY(hipMemcpy(st.data(), job->state.p, job->N * sizeof(uint4), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < job->N; ++i) {
+        if (rgba) memcpy(rgba + i * 4, &st[i].x, 4);
+        if (coord) {
+            coord[i * 3] = st[i].y & 0xffffu;
+            coord[i * 3 + 1] = st[i].y >> 16;
+            coord[i * 3 + 2] = st[i].z & 0xffffu;
+        }
+        if (ids) {
+            ids[i * 2] = st[i].w;
+            ids[i * 2 + 1] = st[i].z >> 16;
+        }
+    }
+    if (resolved_flat || resolved_score) {
+        std::vector<uint32_t> rl(job->n_res);
+        if (job->n_res) HIP_TRY(hipMemcpy(rl.data(), job->rlist.p, job->n_res * 4, hipMemcpyDeviceToHost));
+        std::vector<float> sc(job->N);
+        HIP_TRY(hipMemcpy(sc.data(), job->score.p, job->N * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < job->n_res; ++i) {
+            if (resolved_flat) resolved_flat[i] = rl[i];
+            if (resolved_score) resolved_score[i] = sc[rl[i]];
+        }
+    }
+    return TSG_OK;
+}
+
+int tsg_job_stage_stats(tsg_job* job, double* sums, uint64_t* counts, uint64_t* items, uint32_t cap) {
+    if (!job) return fail(TSG_ERR_INVALID_ARGUMENT, "null job");
+    uint32_t n = std::min<uint32_t>(cap, job->prm.backtrack_stages + 1);
+    std::vector<double> s(job->prm.backtrack_stages + 1);
+    std::vector<unsigned long long> c(job->prm.backtrack_stages + 1);
+    HIP_TRY(hipSetDevice(job->ctx->device));
+    HIP_TRY(hipMemcpy(s.data(), job->st_sum.p, s.size() * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(c.data(), job->st_cnt.p, c.size() * 8, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; ++i) {
+        if (sums) sums[i] = s[i];
+        if (counts) counts[i] = c[i];
+        if (items) items[i] = i < job->st_items.size() ? job->st_items[i] : 0;
+    }
+    return (int)n;
+}
+
+int tsg_job_timing(tsg_job* job, double* run_ms, uint64_t* n_batches) {
+    if (!job) return fail(TSG_ERR_INVALID_ARGUMENT, "null job");
+    if (run_ms) *run_ms = job->run_ms;
+    if (n_batches) *n_batches = job->n_batches;
+    return TSG_OK;
+}
+
+int tsg_job_set_profiling(tsg_job* job, int enable) {
+    if (!job) return fail(TSG_ERR_INVALID_ARGUMENT, "null job");
+    job->profile = enable != 0;
+    return TSG_OK;
+}
+
+int tsg_job_kernel_stats(tsg_job* job, double* ms, uint64_t* items, uint64_t* launches) {
+    if (!job) return fail(TSG_ERR_INVALID_ARGUMENT, "null job");
+    for (int q = 0; q < 5; ++q) {
+        if (ms) ms[q] = job->kern_ms[q];
+        if (items) items[q] = job->kern_items[q];
+        if (launches) launches[q] = job->kern_launches[q];
+    }
+    return TSG_OK;
+}
+
+void tsg_job_destroy(tsg_job* job) {
+    if (!job) return;
+    (void)hipSetDevice(job->ctx->device);
+    (void)hipStreamSynchronize(job->ctx->stream);
+    delete job;
+}
+
+int tsg_pyramid_build(tsg_ctx* ctx, const tsg_image* img, uint32_t levels, uint8_t* out) {
+    if (!ctx || !valid_image(img) || !out) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    uint32_t L = levels == 0 ? 1 : levels;
+    size_t npx = (size_t)img->w * img->h;
+    DBuf<uint32_t> d;
+    HIP_TRY(d.alloc(L * npx));
+    int rc = device_pyramid(img->rgba, img->w, img->h, L, d.p, ctx->stream);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpy(out, d.p, L * npx * 4, hipMemcpyDeviceToHost));
+    return TSG_OK;
+}
+
+int tsg_knn_batch(tsg_ctx* ctx, const uint8_t* mask, uint32_t w, uint32_t h, uint32_t n_q, const uint32_t* q_flat,
+                  uint32_t k, int32_t* nbrs, uint32_t* counts) {
+    if (!ctx || !mask || !q_flat || !nbrs || !counts) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    if (k == 0 || k > (uint32_t)kMaxK) return fail(TSG_ERR_UNSUPPORTED, "k must be in 1..64");
+    if (w == 0 || h == 0 || w > 16384 || h > 16384) return fail(TSG_ERR_INVALID_ARGUMENT, "bad size");
+    for (uint32_t q = 0; q < n_q; ++q)
+        if (q_flat[q] >= w * h) return fail(TSG_ERR_INVALID_ARGUMENT, "query outside the image");
+    if (n_q == 0) return TSG_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    hipStream_t s = ctx->stream;
+    uint32_t tx = (w + 7) / 8, ty = (h + 7) / 8;
+    std::vector<uint64_t> tiles((size_t)tx * ty, 0);
+    std::vector<uint32_t> rl;
+    for (uint32_t i = 0; i < w * h; ++i)
+        if (mask[i]) {
+            uint32_t x = i % w, y = i / w;
+            tiles[(y >> 3) * tx + (x >> 3)] |= 1ull << (((y & 7u) << 3) | (x & 7u));
+            rl.push_back(i);
+        }
+    DBuf<uint64_t> dm;
+    DBuf<uint32_t> drl, dq, dcnt;
+    DBuf<int32_t> dn;
+    DBuf<uint4> dres;
+    HIP_TRY(dm.alloc(tiles.size()));
+    HIP_TRY(drl.alloc(rl.size()));
+    HIP_TRY(dq.alloc(n_q));
+    HIP_TRY(dcnt.alloc(n_q));
+    HIP_TRY(dn.alloc((size_t)n_q * k));
+    HIP_TRY(dres.alloc(n_q));
+    HIP_TRY(hipMemcpy(dm.p, tiles.data(), tiles.size() * 8, hipMemcpyHostToDevice));
+    if (!rl.empty()) HIP_TRY(hipMemcpy(drl.p, rl.data(), rl.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dq.p, q_flat, n_q * 4, hipMemcpyHostToDevice));
+    BatchArgs a{};
+    a.W = w;
+    a.H = h;
+    a.tiles_x = tx;
+    a.tiles_y = ty;
+    a.mask = dm.p;
+    a.work = dq.p;
+    a.rlist = drl.p;
+    a.n_res = (uint32_t)rl.size();
+    a.offtab = ctx->offtab.p;
+    a.offtab_n = ctx->offtab_n;
+    a.lut = ctx->dummy_lut.p;
+    a.item0 = 0;
+    a.item1 = n_q;
+    a.k = k;
+    a.result = dres.p;
+    a.dbg_nbrs = dn.p;
+    a.dbg_nbr_count = dcnt.p;
+    a.knn_only = 1;
+    HIP_TRY(launch_resolve(a, false, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    std::vector<int32_t> packedn((size_t)n_q * k);
+    HIP_TRY(hipMemcpy(packedn.data(), dn.p, packedn.size() * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(counts, dcnt.p, n_q * 4, hipMemcpyDeviceToHost));
+    for (uint32_t q = 0; q < n_q; ++q) {
+        int x = (int)(q_flat[q] % w), y = (int)(q_flat[q] / w);
+        for (uint32_t jn = 0; jn < k; ++jn) {
+            int32_t o = packedn[(size_t)q * k + jn];
+            bool v = jn < counts[q];
+            nbrs[((size_t)q * k + jn) * 2] = v ? x + (int)(int16_t)(o & 0xffff) : 0;
+            nbrs[((size_t)q * k + jn) * 2 + 1] = v ? y + (o >> 16) : 0;
+        }
+    }
+    return TSG_OK;
+}
+
+int tsg_random_candidates(tsg_ctx* ctx, uint32_t n_maps, const tsg_image* maps, const tsg_image* sample_masks,
+                          uint64_t stage_seed, uint32_t item0, uint32_t n, uint32_t m, uint32_t* out) {
+    if (!ctx || !maps || !out || n_maps == 0) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    if (n == 0 || m == 0) return TSG_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    std::vector<std::unique_ptr<DBuf<uint8_t>>> keep;
+    std::vector<MapDesc> md(n_maps);
+    for (uint32_t i = 0; i < n_maps; ++i) {
+        md[i] = MapDesc{};
+        md[i].w = maps[i].w;
+        md[i].h = maps[i].h;
+        if (sample_masks && sample_masks[i].rgba) {
+            std::vector<uint8_t> v((size_t)maps[i].w * maps[i].h);
+            size_t valid = 0;
+            for (uint32_t y = 0; y < maps[i].h; ++y)
+                for (uint32_t x = 0; x < maps[i].w; ++x) {
+                    uint8_t r = sample_masks[i].rgba[((size_t)y * sample_masks[i].w + x) * 4];
+                    v[(size_t)y * maps[i].w + x] = r != 0;
+                    valid += r != 0;
+                }
+            if (!valid) return fail(TSG_ERR_SAMPLING, "sample mask admits no pixel");
+            keep.emplace_back(new DBuf<uint8_t>());
+            HIP_TRY(keep.back()->alloc(v.size()));
+            HIP_TRY(hipMemcpy(keep.back()->p, v.data(), v.size(), hipMemcpyHostToDevice));
+            md[i].smask = keep.back()->p;
+        }
+    }
+    DBuf<MapDesc> dmd;
+    DBuf<uint2> dout;
+    DBuf<uint32_t> ovf;
+    HIP_TRY(dmd.alloc(n_maps));
+    HIP_TRY(dout.alloc((size_t)n * m));
+    HIP_TRY(ovf.alloc(1));
+    HIP_TRY(hipMemset(ovf.p, 0, 4));
+    HIP_TRY(hipMemcpy(dmd.p, md.data(), n_maps * sizeof(MapDesc), hipMemcpyHostToDevice));
+    RandArgs ra{dmd.p, n_maps, stage_seed, item0, item0 + n, m, dout.p, ovf.p};
+    HIP_TRY(launch_rand(ra, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    std::vector<uint2> h((size_t)n * m);
+    HIP_TRY(hipMemcpy(h.data(), dout.p, h.size() * 8, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < h.size(); ++i) {
+        out[i * 3] = h[i].x & 0xffffu;
+        out[i * 3 + 1] = h[i].x >> 16;
+        out[i * 3 + 2] = h[i].y;
+    }
+    return TSG_OK;
+}
+
+int tsg_score_batch(tsg_ctx* ctx, uint32_t w, uint32_t h, const uint8_t* color, const uint32_t* coord,
+                    const uint32_t* ids, uint32_t n_maps, const tsg_image* maps, const tsg_image* guides,
+                    const tsg_image* target_guide, const float* lut_color, const float* lut_guide, uint32_t k,
+                    uint32_t cmax, uint32_t n, const uint32_t* flat, const uint32_t* nbr_count, const int32_t* nbrs,
+                    const uint32_t* cand_count, const int32_t* cands, float* full_cost, uint32_t* best,
+                    float* best_score) {
+    if (!ctx || !color || !coord || !ids || !maps || !lut_color || !flat || !nbr_count || !nbrs || !cand_count ||
+        !cands || !full_cost || !best || !best_score)
+        return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    if (k == 0 || k > (uint32_t)kMaxK) return fail(TSG_ERR_UNSUPPORTED, "k must be in 1..64");
+    bool guided = guides != nullptr;
+    if (guided && (!target_guide || !lut_guide)) return fail(TSG_ERR_INVALID_ARGUMENT, "guided needs all guides");
+    if (n == 0) return TSG_OK;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (flat[i] >= w * h || nbr_count[i] > k || cand_count[i] > cmax || nbr_count[i] == 0 || cand_count[i] == 0)
+            return fail(TSG_ERR_INVALID_ARGUMENT, "bad item");
+        for (uint32_t c = 0; c < cand_count[i]; ++c) {
+            const int32_t* e = cands + ((size_t)i * cmax + c) * 6;
+            if ((uint32_t)e[2] >= n_maps || e[0] < 0 || e[1] < 0 || e[0] > 65535 || e[1] > 65535)
+                return fail(TSG_ERR_INVALID_ARGUMENT, "bad candidate");
+        }
+        for (uint32_t jn = 0; jn < nbr_count[i]; ++jn) {
+            int nx = nbrs[((size_t)i * k + jn) * 2], ny = nbrs[((size_t)i * k + jn) * 2 + 1];
+            if (nx < 0 || ny < 0 || (uint32_t)nx >= w || (uint32_t)ny >= h)
+                return fail(TSG_ERR_INVALID_ARGUMENT, "neighbour outside the image");
+        }
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    hipStream_t s = ctx->stream;
+    size_t N = (size_t)w * h;
+    std::vector<uint4> st(N);
+    for (size_t i = 0; i < N; ++i)
+        st[i] = make_uint4(pack_px(color + i * 4), coord[i * 3] | (coord[i * 3 + 1] << 16),
+                           coord[i * 3 + 2] | (ids[i * 2 + 1] << 16), ids[i * 2]);
+    std::vector<std::unique_ptr<DBuf<uint32_t>>> keep;
+    auto upload_img = [&](const tsg_image& im) -> uint32_t* {
+        std::vector<uint32_t> v = packed(im);
+        keep.emplace_back(new DBuf<uint32_t>());
+        if (keep.back()->alloc(v.size()) != hipSuccess) return nullptr;
+        if (hipMemcpy(keep.back()->p, v.data(), v.size() * 4, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+        return keep.back()->p;
+    };
+    std::vector<MapDesc> md(n_maps);
+    for (uint32_t i = 0; i < n_maps; ++i) {
+        md[i] = MapDesc{};
+        md[i].w = maps[i].w;
+        md[i].h = maps[i].h;
+        if (!(md[i].px = upload_img(maps[i]))) return fail(TSG_ERR_DEVICE, "upload failed");
+        if (guided) {
+            md[i].gw = guides[i].w;
+            md[i].gh = guides[i].h;
+            if (!(md[i].gpx = upload_img(guides[i]))) return fail(TSG_ERR_DEVICE, "upload failed");
+        }
+    }
+    uint32_t* dtg = nullptr;
+    if (guided && !(dtg = upload_img(*target_guide))) return fail(TSG_ERR_DEVICE, "upload failed");
+    std::vector<float> lut(512, 0.0f);
+    memcpy(lut.data(), lut_color, 256 * 4);
+    if (guided) memcpy(lut.data() + 256, lut_guide, 256 * 4);
+    std::vector<int32_t> pn((size_t)n * k, 0);
+    std::vector<uint4> pc((size_t)n * cmax, make_uint4(0, 0, 0, 0));
+    for (uint32_t i = 0; i < n; ++i) {
+        int x = (int)(flat[i] % w), y = (int)(flat[i] / w);
+        for (uint32_t jn = 0; jn < nbr_count[i]; ++jn) {
+            int dx = nbrs[((size_t)i * k + jn) * 2] - x, dy = nbrs[((size_t)i * k + jn) * 2 + 1] - y;
+            pn[(size_t)i * k + jn] = (int32_t)(((uint32_t)dx & 0xffffu) | ((uint32_t)dy << 16));
+        }
+        for (uint32_t c = 0; c < cand_count[i]; ++c) {
+            const int32_t* e = cands + ((size_t)i * cmax + c) * 6;
+            pc[(size_t)i * cmax + c] =
+                make_uint4((uint32_t)e[0] | ((uint32_t)e[1] << 16), (uint32_t)e[2] | ((uint32_t)(e[3] != 0) << 31),
+                           (uint32_t)e[4], 0u);
+        }
+    }
+    DBuf<uint4> dst, dpc, dres;
+    DBuf<MapDesc> dmd;
+    DBuf<float> dlut, dfc;
+    DBuf<uint32_t> dflat, dnc, dcc, dbest;
+    DBuf<int32_t> dpn;
+    HIP_TRY(dst.alloc(N));
+    HIP_TRY(dpc.alloc(pc.size()));
+    HIP_TRY(dres.alloc(n));
+    HIP_TRY(dmd.alloc(n_maps));
+    HIP_TRY(dlut.alloc(512));
+    HIP_TRY(dfc.alloc((size_t)n * cmax));
+    HIP_TRY(dflat.alloc(n));
+    HIP_TRY(dnc.alloc(n));
+    HIP_TRY(dcc.alloc(n));
+    HIP_TRY(dbest.alloc(n));
+    HIP_TRY(dpn.alloc(pn.size()));
+    HIP_TRY(hipMemcpy(dst.p, st.data(), N * sizeof(uint4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dpc.p, pc.data(), pc.size() * sizeof(uint4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dmd.p, md.data(), n_maps * sizeof(MapDesc), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dlut.p, lut.data(), 512 * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dflat.p, flat, n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dnc.p, nbr_count, n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dcc.p, cand_count, n * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dpn.p, pn.data(), pn.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(dfc.p, 0, (size_t)n * cmax * 4));
+    BatchArgs a{};
+    a.W = w;
+    a.H = h;
+    a.tiles_x = (w + 7) / 8;
+    a.tiles_y = (h + 7) / 8;
+    a.state = dst.p;
+    a.tg = dtg;
+    a.maps = dmd.p;
+    a.n_maps = n_maps;
+    a.work = dflat.p;
+    a.lut = dlut.p;
+    a.item0 = 0;
+    a.item1 = n;
+    a.k = k;
+    a.result = dres.p;
+    a.ext_nbrs = dpn.p;
+    a.ext_nbr_count = dnc.p;
+    a.ext_cands = dpc.p;
+    a.ext_cand_count = dcc.p;
+    a.ext_cmax = cmax;
+    a.ext_full_cost = dfc.p;
+    a.dbg_best = dbest.p;
+    HIP_TRY(launch_resolve(a, guided, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    std::vector<uint4> res(n);
+    HIP_TRY(hipMemcpy(res.data(), dres.p, n * sizeof(uint4), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(full_cost, dfc.p, (size_t)n * cmax * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(best, dbest.p, n * 4, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; ++i) memcpy(&best_score[i], &res[i].w, 4);
+    return TSG_OK;
+}
+
+}  // extern "C"
