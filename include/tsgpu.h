@@ -127,8 +127,16 @@ int tsg_job_stage_stats(tsg_job* job, double* sums, uint64_t* counts, uint64_t* 
 /* Device time of the last tsg_job_run in milliseconds, and batch count. */
 int tsg_job_timing(tsg_job* job, double* run_ms, uint64_t* n_batches);
 /* Per-kernel device timing of the next tsg_job_run (HIP events on the job's
- * stream around every launch). */
-int tsg_job_set_profiling(tsg_job* job, int enable);
+ * stream around every launch). Level 2 also records, for each work item of
+ * the first stage, device timestamps (100 MHz wall clock) at the phase
+ * boundaries of its resolve: start, kNN, weights, coherence dedup,
+ * coherence costs, random costs, end. */
+int tsg_job_set_profiling(tsg_job* job, int level);
+/* Phase record of the first n_items work items of the first stage (16 u64
+ * each) after a level-2 profiled run: [0..6] the phase timestamps above,
+ * [7] kNN search done before its final selection, [8] ring-search steps,
+ * [9] keys buffered before the final selection. */
+int tsg_job_phase_clocks(tsg_job* job, uint64_t* out, uint64_t n_items);
 /* After a profiled run, per kernel [resolve, commit, random candidates,
  * recolour, stats]: summed milliseconds, items processed, launches.
  * Each array holds 5 entries. */
@@ -165,13 +173,16 @@ int tsg_random_candidates(tsg_ctx* ctx, uint32_t n_maps, const tsg_image* maps, 
  * per item i: flat[i]; nbr_count[i] and nbrs[i*k + j] = (x, y);
  * cand_count[i] and cands[(i*cmax + c)*6] = (x, y, map, kind 0=coherence
  * 1=random, patch, patch map).
- * outputs: full_cost[i*cmax + c] (no early exit), best[i], best_score[i]. */
+ * outputs: full_cost[i*cmax + c] (no early exit), best[i], best_score[i]
+ * from the unpruned evaluation; best_pruned / best_score_pruned (nullable)
+ * from the production evaluation (coherence dedup + exact pruning).
+ * Coherence candidates (kind 0, at most 64) must precede random ones. */
 int tsg_score_batch(tsg_ctx* ctx, uint32_t w, uint32_t h, const uint8_t* color, const uint32_t* coord,
                     const uint32_t* ids, uint32_t n_maps, const tsg_image* maps, const tsg_image* guides,
                     const tsg_image* target_guide, const float* lut_color, const float* lut_guide, uint32_t k,
                     uint32_t cmax, uint32_t n, const uint32_t* flat, const uint32_t* nbr_count, const int32_t* nbrs,
                     const uint32_t* cand_count, const int32_t* cands, float* full_cost, uint32_t* best,
-                    float* best_score);
+                    float* best_score, uint32_t* best_pruned, float* best_score_pruned);
 
 #ifdef __cplusplus
 }

@@ -150,13 +150,17 @@ def _score_on_device(ctx, info, rgba, coord, ids, my, gl, d, examples, k, sample
                           d["cands"], guides=guides, target_guide=tg, lut_guide=gl if guides else None, ctx=ctx)
 
 
-def _check_scores(d, full, best, bs):
+def _check_scores(d, full, best, bs, bp, bsp):
     ok = d["best"] != 0xFFFFFFFF
     for i in np.nonzero(ok)[0]:
         nc = d["cand_count"][i]
         assert np.array_equal(full[i, :nc].view(np.uint32), d["full_cost"][i, :nc].view(np.uint32)), i
+    # unpruned lane-per-candidate evaluation
     assert np.array_equal(best[ok], d["best"][ok])
     assert np.array_equal(bs[ok].view(np.uint32), d["best_score"][ok].view(np.uint32))
+    # production evaluation: coherence dedup + exact pruning gives the same winner
+    assert np.array_equal(bp[ok], d["best"][ok])
+    assert np.array_equal(bsp[ok].view(np.uint32), d["best_score"][ok].view(np.uint32))
 
 
 @pytest.mark.parametrize("stage_t,item", [(4, 0), (4, 16384), (2, 3000)])
@@ -164,8 +168,8 @@ def test_frozen_state_scoring_bit_exact_256(ctx, stage_t, item):
     exs = [synth.example(211, 128, 128)]
     job, info, rgba, coord, ids, my, gl, d = _frozen(exs, 256, stage_t, item, 4096, seed=211)
     assert len(d["flat"]) > 100
-    full, best, bs = _score_on_device(ctx, info, rgba, coord, ids, my, gl, d, exs, 50)
-    _check_scores(d, full, best, bs)
+    full, best, bs, bp, bsp = _score_on_device(ctx, info, rgba, coord, ids, my, gl, d, exs, 50)
+    _check_scores(d, full, best, bs, bp, bsp)
 
 
 def test_frozen_state_scoring_guided_multi_example(ctx):
@@ -175,9 +179,9 @@ def test_frozen_state_scoring_guided_multi_example(ctx):
     job, info, rgba, coord, ids, my, gl, d = _frozen(exs, 96, 3, 500, 2048, seed=211, example_guides=gds,
                                                      target_guide=tg)
     assert info["alpha"] > 0
-    full, best, bs = _score_on_device(ctx, info, rgba, coord, ids, my, gl, d, exs, 50, example_guides=gds,
+    full, best, bs, bp, bsp = _score_on_device(ctx, info, rgba, coord, ids, my, gl, d, exs, 50, example_guides=gds,
                                       target_guide=tg)
-    _check_scores(d, full, best, bs)
+    _check_scores(d, full, best, bs, bp, bsp)
 
 
 def test_random_candidates_bit_exact(ctx):
@@ -200,11 +204,11 @@ def test_frozen_state_scoring_bit_exact_1024(ctx):
     # the C2 shape: 512^2 example -> 1024^2, largest stage, 4096 items per batch
     exs = [synth.example(512, 512, 512)]
     job, info, rgba, coord, ids, my, gl, d = _frozen(exs, 1024, 5, 0, 4096, threads=16, seed=0)
-    full, best, bs = _score_on_device(ctx, info, rgba, coord, ids, my, gl, d, exs, 50)
-    _check_scores(d, full, best, bs)
+    full, best, bs, bp, bsp = _score_on_device(ctx, info, rgba, coord, ids, my, gl, d, exs, 50)
+    _check_scores(d, full, best, bs, bp, bsp)
     job2, info2, rgba2, coord2, ids2, my2, gl2, d2 = _frozen(exs, 1024, 5, 600000, 4096, threads=16, seed=0)
-    full, best, bs = _score_on_device(ctx, info2, rgba2, coord2, ids2, my2, gl2, d2, exs, 50)
-    _check_scores(d2, full, best, bs)
+    full, best, bs, bp, bsp = _score_on_device(ctx, info2, rgba2, coord2, ids2, my2, gl2, d2, exs, 50)
+    _check_scores(d2, full, best, bs, bp, bsp)
 
 
 # ------------------------------------------------------------------ whole images

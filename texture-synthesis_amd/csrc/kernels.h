@@ -43,6 +43,16 @@ struct BatchArgs {
     uint32_t* dbg_nbr_count;
     uint32_t* dbg_best;           // [item] winning candidate index (test hook)
     uint32_t knn_only;            // stop after the kNN dump (test hook)
+    uint32_t exact;               // force the unpruned lane-per-candidate evaluation
+    uint32_t lut_nan;             // the stage's cost tables hold a NaN (cauchy 0): exact path
+    // commit fused into the resolve launch (last workgroup) / small-batch kernel
+    uint32_t fused_commit;
+    uint32_t first_new;           // items >= first_new are new pixels
+    uint32_t* done_counter;       // zero between launches
+    uint4* state_w;
+    uint64_t* mask_w;
+    float* score_w;
+    unsigned long long* dbg_clk;  // [stage item][8] phase timestamps (profiling)
 };
 
 struct RandArgs {
@@ -84,6 +94,11 @@ struct StatsArgs {
 
 hipError_t configure_kernels();
 size_t resolve_lds_bytes(bool guided);
+size_t resolve_small_lds_bytes(bool guided);
+uint32_t small_batch_max();
+// consecutive batches of at most small_batch_max() items, one workgroup
+hipError_t launch_resolve_small(const BatchArgs& a, const uint32_t* bounds, const uint32_t* nres,
+                                uint32_t n_batches, bool guided, hipStream_t s);
 hipError_t launch_resolve(const BatchArgs& a, bool guided, hipStream_t s);
 hipError_t launch_stats(const StatsArgs& a, hipStream_t s);
 hipError_t launch_rand(const RandArgs& a, hipStream_t s);

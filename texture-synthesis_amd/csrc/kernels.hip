@@ -24,6 +24,7 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kWavesPerWG = 8;
+constexpr int kSmallWaves = 16;  // items per batch of k_resolve_small
 constexpr int kKeyCap = 256;
 constexpr int kLutCopies = 32;
 
@@ -54,14 +55,26 @@ __device__ __forceinline__ int32_t key_to_packed(uint64_t key) {
 __device__ __forceinline__ int packed_dx(int32_t o) { return (int)(int16_t)(o & 0xffff); }
 __device__ __forceinline__ int packed_dy(int32_t o) { return o >> 16; }
 
+constexpr uint32_t kNone = 0xffffffffu;
+constexpr int kTermCap = 512;  // terms per chunk of the pruned evaluation
+
 struct WaveLds {
-    uint64_t keys[kKeyCap];
+    union {
+        uint64_t keys[kKeyCap];   // kNN selection
+        float terms[kTermCap];    // per-chunk cost terms (candidate-major)
+    };
+    static_assert(kTermCap * 4 <= kKeyCap * 8, "terms share the key buffer");
     double dist[kMaxK];
     int32_t nd[kMaxK];
     float w[kMaxK];
     uint32_t pat[kMaxK];
     uint32_t gpat[kMaxK];
     uint4 coh[kMaxK];
+    // active candidates of the pruned evaluation
+    uint32_t act_xy[kWave];
+    uint32_t act_ms[kWave];   // map | 1 << 31 for mirrored (random) patterns
+    uint32_t act_idx[kWave];  // index in the reference's candidate list
+    float act_sum[kWave];
     double avg;
 };
 
@@ -71,10 +84,11 @@ struct Sel {
     uint64_t kth;  // current k-th smallest key once `count` reached k
 };
 
-__device__ void wave_sort(uint64_t* keys, int lane) {
-    for (int size = 2; size <= kKeyCap; size <<= 1) {
+__device__ __noinline__ void wave_sort(uint64_t* keys, int lane, int n) {
+    // bitonic sort of keys[0, n), n a power of two in [2, kKeyCap]
+    for (int size = 2; size <= n; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int t = lane; t < kKeyCap / 2; t += kWave) {
+            for (int t = lane; t < n / 2; t += kWave) {
                 int i = ((t & ~(stride - 1)) << 1) | (t & (stride - 1));
                 int j = i + stride;
                 uint64_t a = keys[i], b = keys[j];
@@ -89,13 +103,81 @@ __device__ void wave_sort(uint64_t* keys, int lane) {
     }
 }
 
-__device__ void sel_compact(uint64_t* keys, Sel& s, uint32_t k, int lane) {
-    for (int t = lane; t < kKeyCap; t += kWave)
-        if ((uint32_t)t >= s.count) keys[t] = ~0ull;
+// Bitonic sort of one u64 key per lane, ascending in lane order.
+__device__ __forceinline__ uint64_t wave_sort64(uint64_t v, int lane) {
+    for (int size = 2; size <= kWave; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const uint64_t o = ((uint64_t)(uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), stride) << 32) |
+                               (uint32_t)__shfl_xor((int)(uint32_t)v, stride);
+            const bool up = (lane & size) == 0 || size == kWave;
+            const bool keep_min = ((lane & stride) == 0) == up;
+            v = keep_min ? (o < v ? o : v) : (o > v ? o : v);
+        }
+    }
+    return v;
+}
+
+// Keeps the k smallest buffered keys, sorted, in keys[0, min(count, k)).
+// Up to 64 keys: one register sort. More: the k-th smallest squared
+// distance T is found by bisection (ballot counts), the keys with d^2 <= T
+// (k plus ties) are compacted and register-sorted; a tie group that pushes
+// them past 64 takes the full LDS sort instead.
+__device__ __noinline__ void sel_compact(uint64_t* keys, Sel& s, uint32_t k, int lane) {
+    const uint32_t cnt = s.count;
+    if (cnt <= (uint32_t)kWave) {
+        uint64_t v = (uint32_t)lane < cnt ? keys[lane] : ~0ull;
+        v = wave_sort64(v, lane);
+        wave_sync();
+        keys[lane] = v;
+        wave_sync();
+        s.count = cnt < k ? cnt : k;
+        s.kth = s.count == k ? keys[k - 1] : ~0ull;
+        return;
+    }
+    uint64_t v[kKeyCap / kWave];
+    uint32_t hi = 0;
+#pragma unroll
+    for (int r = 0; r < kKeyCap / kWave; ++r) {
+        const uint32_t idx = (uint32_t)(lane + r * kWave);
+        v[r] = idx < cnt ? keys[idx] : ~0ull;
+        if (idx < cnt) hi = max(hi, (uint32_t)(v[r] >> 32));
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) hi = max(hi, (uint32_t)__shfl_xor((int)hi, off));
+    uint32_t lo = 0;
+    while (lo < hi) {  // smallest T with #{d^2 <= T} >= k
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        uint32_t c = 0;
+#pragma unroll
+        for (int r = 0; r < kKeyCap / kWave; ++r) c += (uint32_t)__popcll(__ballot((uint32_t)(v[r] >> 32) <= mid));
+        if (c >= k) hi = mid;
+        else lo = mid + 1;
+    }
+    uint32_t n_le = 0;
+#pragma unroll
+    for (int r = 0; r < kKeyCap / kWave; ++r) n_le += (uint32_t)__popcll(__ballot((uint32_t)(v[r] >> 32) <= lo));
     wave_sync();
-    wave_sort(keys, lane);
-    s.count = s.count < k ? s.count : k;
-    s.kth = s.count == k ? keys[k - 1] : ~0ull;
+    if (n_le <= (uint32_t)kWave) {
+        uint32_t base = 0;
+#pragma unroll
+        for (int r = 0; r < kKeyCap / kWave; ++r) {
+            const bool sel = (uint32_t)(v[r] >> 32) <= lo;
+            const uint64_t m = __ballot(sel);
+            if (sel) keys[base + __popcll(m & lanemask_lt(lane))] = v[r];
+            base += (uint32_t)__popcll(m);
+        }
+        wave_sync();
+        uint64_t w = (uint32_t)lane < n_le ? keys[lane] : ~0ull;
+        w = wave_sort64(w, lane);
+        keys[lane] = w;
+        wave_sync();
+    } else {
+        for (int t = lane; t < kKeyCap; t += kWave)
+            if ((uint32_t)t >= cnt) keys[t] = ~0ull;
+        wave_sync();
+        wave_sort(keys, lane, kKeyCap);
+    }
+    s.count = k;
+    s.kth = keys[k - 1];
 }
 
 __device__ __forceinline__ void sel_push(uint64_t* keys, Sel& s, uint32_t k, bool valid, uint64_t key, int lane) {
@@ -115,75 +197,118 @@ __device__ __forceinline__ bool px_set(const uint64_t* mask, uint32_t tiles_x, i
 }
 
 // Exhaustive scan of the resolved list (sparse phase).
-__device__ void knn_list(const BatchArgs& a, WaveLds* L, int x, int y, uint32_t k, int lane, Sel& s) {
-    for (uint32_t base = 0; base < a.n_res; base += kWave) {
-        uint32_t idx = base + lane;
-        bool v = idx < a.n_res;
-        uint64_t key = ~0ull;
-        if (v) {
-            uint32_t f = a.rlist[idx];
-            int X = (int)(f % a.W), Y = (int)(f / a.W);
-            key = make_key(X - x, Y - y);
+__device__ __forceinline__ void knn_list(const BatchArgs& a, WaveLds* L, uint32_t n_res, int x, int y, uint32_t k, int lane, Sel& s) {
+    // four chunks of the list per memory round trip
+    for (uint32_t base = 0; base < n_res; base += 4 * kWave) {
+        uint32_t f[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t idx = base + u * kWave + lane;
+            f[u] = idx < n_res ? a.rlist[idx] : 0u;
         }
-        sel_push(L->keys, s, k, v, key, lane);
+#pragma unroll 1
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t idx = base + u * kWave + lane;
+            if (base + u * kWave >= n_res) break;
+            const bool v = idx < n_res;
+            uint64_t key = ~0ull;
+            if (v) key = make_key((int)(f[u] % a.W) - x, (int)(f[u] / a.W) - y);
+            sel_push(L->keys, s, k, v, key, lane);
+        }
     }
 }
 
-// Ring search over 8x8 mask tiles, nearest tile rings first; stops once the
-// k-th best distance is below the distance to any unvisited tile.
-__device__ void knn_rings(const BatchArgs& a, WaveLds* L, int x, int y, uint32_t k, int lane, Sel& s) {
+// Integer square root of small non-negative t.
+__device__ __forceinline__ int isqrt_small(int t) {
+    int r = (int)sqrtf((float)t);
+    while (r * r > t) --r;
+    while ((r + 1) * (r + 1) <= t) ++r;
+    return r;
+}
+
+// Tile offset of spiral index t: ring 0 is the query's tile; ring r >= 1
+// holds the 8r tiles at Chebyshev distance r (top, right, bottom, left).
+__device__ __forceinline__ void spiral_tile(int t, int& ox, int& oy) {
+    if (t == 0) {
+        ox = oy = 0;
+        return;
+    }
+    const int r = (isqrt_small(t) + 1) >> 1;
+    const int o = t - (2 * r - 1) * (2 * r - 1);
+    const int side = o / (2 * r), off = o - side * (2 * r);
+    if (side == 0) {
+        ox = -r + off;
+        oy = -r;
+    } else if (side == 1) {
+        ox = r;
+        oy = -r + off;
+    } else if (side == 2) {
+        ox = r - off;
+        oy = r;
+    } else {
+        ox = -r;
+        oy = r - off;
+    }
+}
+
+// Ring search over the 8x8-tile resolved mask: 64 tiles per step in spiral
+// order, one tile per lane, the set bits of each tile taken lane-parallel.
+// Stops when k collected points lie closer than any pixel outside the
+// completed rings (every point that close has been collected).
+__device__ __forceinline__ void knn_rings(const BatchArgs& a, WaveLds* L, int x, int y, uint32_t k, int lane, Sel& s,
+                                          uint32_t& iters) {
     const int tx = x >> 3, ty = y >> 3;
     const int TX = (int)a.tiles_x, TY = (int)a.tiles_y;
-    for (int r = 0;; ++r) {
-        int nt = r == 0 ? 1 : 8 * r;
-        for (int tb = 0; tb < nt; tb += kWave) {
-            int t = tb + lane;
-            uint64_t word = 0;
-            int gx = 0, gy = 0;
-            if (t < nt) {
-                int ox = 0, oy = 0;
-                if (r > 0) {
-                    int side = t / (2 * r), o = t - side * (2 * r);
-                    if (side == 0) {
-                        ox = -r + o;
-                        oy = -r;
-                    } else if (side == 1) {
-                        ox = r;
-                        oy = -r + o;
-                    } else if (side == 2) {
-                        ox = r - o;
-                        oy = r;
-                    } else {
-                        ox = -r;
-                        oy = r - o;
-                    }
-                }
-                gx = tx + ox;
-                gy = ty + oy;
-                if (gx >= 0 && gy >= 0 && gx < TX && gy < TY) word = a.mask[(uint32_t)gy * a.tiles_x + (uint32_t)gx];
-            }
-            uint64_t nz = __ballot(word != 0);
-            while (nz) {
-                int l = __ffsll((long long)nz) - 1;
-                nz &= nz - 1;
-                uint64_t wd = shfl_u64(word, l);
-                int bx = __shfl(gx, l), by = __shfl(gy, l);
-                bool b = (wd >> lane) & 1ull;
-                int X = bx * 8 + (lane & 7), Y = by * 8 + (lane >> 3);
-                sel_push(L->keys, s, k, b, make_key(X - x, Y - y), lane);
+    const int rmax = max(max(tx, TX - 1 - tx), max(ty, TY - 1 - ty));
+    const int t_end = (2 * rmax + 1) * (2 * rmax + 1);
+    iters = 0;
+    for (int t0 = 0; t0 < t_end; t0 += 4 * kWave) {
+        ++iters;
+        // 256 tiles per memory round trip, then their set bits lane-parallel
+        uint64_t word[4];
+        int gx[4], gy[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int t = t0 + u * kWave + lane;
+            word[u] = 0;
+            gx[u] = gy[u] = 0;
+            if (t < t_end) {
+                int ox, oy;
+                spiral_tile(t, ox, oy);
+                gx[u] = tx + ox;
+                gy[u] = ty + oy;
+                if (gx[u] >= 0 && gy[u] >= 0 && gx[u] < TX && gy[u] < TY)
+                    word[u] = a.mask[(uint32_t)gy[u] * a.tiles_x + (uint32_t)gx[u]];
             }
         }
-        bool all = (tx - r <= 0) && (ty - r <= 0) && (tx + r >= TX - 1) && (ty + r >= TY - 1);
-        if (all) break;
-        if (s.count >= k) {
-            wave_sync();
-            sel_compact(L->keys, s, k, lane);
+#pragma unroll 1
+        for (int u = 0; u < 4; ++u) {
+            uint64_t rest = word[u];
+            while (__ballot(rest != 0)) {
+                const bool have = rest != 0;
+                uint64_t key = ~0ull;
+                if (have) {
+                    const int b = __ffsll((long long)rest) - 1;
+                    rest &= rest - 1;
+                    key = make_key(gx[u] * 8 + (b & 7) - x, gy[u] * 8 + (b >> 3) - y);
+                }
+                sel_push(L->keys, s, k, have, key, lane);
+            }
+        }
+        // rings fully visited after these tiles
+        const int rc = (isqrt_small(min(t0 + 4 * kWave, t_end)) - 1) >> 1;
+        if (rc >= rmax) break;
+        if (s.count >= k && rc >= 0) {
             int64_t D = INT64_MAX, e;
-            if (tx - r > 0 && (e = x - (tx - r) * 8 + 1) < D) D = e;
-            if (tx + r < TX - 1 && (e = (tx + r + 1) * 8 - x) < D) D = e;
-            if (ty - r > 0 && (e = y - (ty - r) * 8 + 1) < D) D = e;
-            if (ty + r < TY - 1 && (e = (ty + r + 1) * 8 - y) < D) D = e;
-            if ((int64_t)(s.kth >> 32) < D * D) break;
+            if (tx - rc > 0 && (e = x - (tx - rc) * 8 + 1) < D) D = e;
+            if (tx + rc < TX - 1 && (e = (tx + rc + 1) * 8 - x) < D) D = e;
+            if (ty - rc > 0 && (e = y - (ty - rc) * 8 + 1) < D) D = e;
+            if (ty + rc < TY - 1 && (e = (ty + rc + 1) * 8 - y) < D) D = e;
+            const uint64_t lim = (uint64_t)(D * D) << 32;
+            uint32_t c = 0;
+            for (int q = lane; q < (int)s.count; q += kWave) c += L->keys[q] < lim;
+            for (int off = kWave / 2; off > 0; off >>= 1) c += (uint32_t)__shfl_xor((int)c, off);
+            if (c >= k) break;
         }
     }
 }
@@ -191,7 +316,7 @@ __device__ void knn_rings(const BatchArgs& a, WaveLds* L, int x, int y, uint32_t
 // Ring-ordered offset table scan: the first k resolved pixels in table
 // order are the k nearest (the table lists every offset within its radius
 // in canonical order). Returns the number found (<= k).
-__device__ uint32_t knn_table(const BatchArgs& a, WaveLds* L, int x, int y, uint32_t k, int lane) {
+__device__ __forceinline__ uint32_t knn_table(const BatchArgs& a, WaveLds* L, int x, int y, uint32_t k, int lane) {
     uint32_t found = 0;
     for (uint32_t base = 0; base < a.offtab_n; base += kWave) {
         uint32_t t = base + lane;
@@ -229,26 +354,164 @@ __device__ __forceinline__ float lut4(const float* lut, uint32_t q, uint32_t p, 
     return s;
 }
 
-template <bool GUIDED>
-__global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve(BatchArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    float* lut = reinterpret_cast<float*>(smem);
-    float* glut = lut + 256 * kLutCopies;
-    WaveLds* wls = reinterpret_cast<WaveLds*>(smem + (GUIDED ? 2 : 1) * 256 * kLutCopies * sizeof(float));
-
-    for (int t = threadIdx.x; t < 256 * kLutCopies; t += blockDim.x) {
-        lut[t] = a.lut[t / kLutCopies];
-        if (GUIDED) glut[t] = a.lut[256 + t / kLutCopies];
+// Candidate ci of the reference's list: valid coherence candidates in
+// neighbour order, then the random ones (find_candidates, ms.rs:496-599).
+__device__ __forceinline__ void cand_get(const BatchArgs& a, const WaveLds* L, uint32_t rel, uint32_t item,
+                                         uint32_t n_coh, uint32_t ci, uint32_t& xy, uint32_t& ms) {
+    if (a.ext_cands) {
+        uint4 e = a.ext_cands[(size_t)rel * a.ext_cmax + ci];
+        xy = e.x;
+        ms = e.y;
+    } else if (ci < n_coh) {
+        uint4 e = L->coh[ci];
+        xy = e.x;
+        ms = e.y;
+    } else {
+        uint2 r = a.rcands[(size_t)(item - a.rc_base) * a.m + (ci - n_coh)];
+        xy = r.x;
+        ms = r.y | 0x80000000u;
     }
-    __syncthreads();
+}
 
-    const int wave = threadIdx.x / kWave;
-    const int lane = threadIdx.x % kWave;
+// Gather of a candidate's pattern pixel for neighbour j (mirrored for
+// random candidates, ms.rs:534/588): colour and, when guided, guide pixel;
+// out-of-bounds pixels are [0,0,0,255] (get_color_of_neighbor, ms.rs:1122-1149).
+template <bool GUIDED>
+__device__ __forceinline__ void gather_px(const MapDesc* maps, const WaveLds* L, uint32_t xy, uint32_t ms, uint32_t j,
+                                          uint32_t& p, uint32_t& gp) {
+    const MapDesc& md = maps[ms & 0x7fffffffu];
+    const int sgn = (ms >> 31) ? -1 : 1;
+    const int32_t o = L->nd[j];
+    const int X2 = (int)(xy & 0xffffu) + sgn * packed_dx(o), Y2 = (int)(xy >> 16) + sgn * packed_dy(o);
+    const bool in = (uint32_t)X2 < md.w && (uint32_t)Y2 < md.h;
+    p = in ? md.px[(uint32_t)Y2 * md.w + (uint32_t)X2] : kOutsidePx;
+    gp = kOutsidePx;
+    if (GUIDED) {
+        const bool gin = (uint32_t)X2 < md.gw && (uint32_t)Y2 < md.gh;
+        gp = gin ? md.gpx[(uint32_t)Y2 * md.gw + (uint32_t)X2] : kOutsidePx;
+    }
+}
+
+// Cost term of neighbour j: (four colour LUT terms, then four guide LUT
+// terms, summed in order) times w_j, as one rounded product.
+template <bool GUIDED>
+__device__ __forceinline__ float term_of(const WaveLds* L, const float* lut, const float* glut, int slot, uint32_t j,
+                                         uint32_t p, uint32_t gp) {
+    float s = lut4(lut, L->pat[j], p, slot);
+    if (GUIDED) {
+        const uint32_t q = L->gpat[j];
+        s = s + glut[abs((int)(q & 255u) - (int)(gp & 255u)) * kLutCopies + slot];
+        s = s + glut[abs((int)((q >> 8) & 255u) - (int)((gp >> 8) & 255u)) * kLutCopies + slot];
+        s = s + glut[abs((int)((q >> 16) & 255u) - (int)((gp >> 16) & 255u)) * kLutCopies + slot];
+        s = s + glut[abs((int)(q >> 24) - (int)(gp >> 24)) * kLutCopies + slot];
+    }
+    return __fmul_rn(s, L->w[j]);
+}
+
+struct Best {
+    float B;      // lowest complete cost so far (FLT_MAX: none)
+    uint32_t Bi;  // its candidate index (kNone: none)
+};
+
+// Evaluates the n_act candidates in L->act_* with exact pruning. Terms of a
+// chunk of neighbours are computed with one lane per (candidate, neighbour)
+// pair; each candidate then adds its terms in neighbour order (the
+// reference's sequential f32 sum). A candidate whose partial sum already
+// exceeds the best complete cost (or equals it with a later index) can
+// never win and is dropped: partial sums of non-negative terms only grow.
+template <bool GUIDED>
+__device__ __forceinline__ void eval_group(const MapDesc* maps, WaveLds* L, const float* lut, const float* glut, int slot, int lane,
+                           uint32_t kf, uint32_t n_act, Best& best) {
+    if (lane < (int)n_act) L->act_sum[lane] = 0.0f;
+    wave_sync();
+    uint32_t j0 = 0;
+    while (n_act > 0) {
+        // chunk of 2^sh neighbours with n_act << sh <= kTermCap; while a bound
+        // exists the first chunk is kept at 8 neighbours so pruning starts early
+        uint32_t sh = (j0 == 0 && best.Bi != kNone) ? 3 : 6;
+        while (sh > 0 && (n_act << sh) > (uint32_t)kTermCap) --sh;
+        const uint32_t jc = 1u << sh;
+        const uint32_t jn = min(jc, kf - j0);
+        const uint32_t total = n_act << sh;
+        // all gathers of the chunk first (at most kTermCap / 64 per lane), then the LUT sums
+        uint32_t px[kTermCap / kWave], gpx[kTermCap / kWave];
+#pragma unroll
+        for (int u = 0; u < kTermCap / kWave; ++u) {
+            const uint32_t l = (uint32_t)(u * kWave + lane), ai = l >> sh, jj = l & (jc - 1);
+            px[u] = gpx[u] = 0;
+            if (l < total && ai < n_act && jj < jn)
+                gather_px<GUIDED>(maps, L, L->act_xy[ai], L->act_ms[ai], j0 + jj, px[u], gpx[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < kTermCap / kWave; ++u) {
+            const uint32_t l = (uint32_t)(u * kWave + lane), ai = l >> sh, jj = l & (jc - 1);
+            if (l < total && ai < n_act && jj < jn) L->terms[l] = term_of<GUIDED>(L, lut, glut, slot, j0 + jj, px[u], gpx[u]);
+        }
+        wave_sync();
+        const bool act = lane < (int)n_act;
+        float s = 0.0f;
+        uint32_t myidx = kNone, xy = 0, ms = 0;
+        if (act) {
+            s = L->act_sum[lane];
+            for (uint32_t jj = 0; jj < jn; ++jj) s = __fadd_rn(s, L->terms[(lane << sh) + jj]);
+            myidx = L->act_idx[lane];
+            xy = L->act_xy[lane];
+            ms = L->act_ms[lane];
+        }
+        j0 += jn;
+        if (j0 >= kf) {
+            float key = act ? s : INFINITY;
+            uint32_t ki = myidx;
+            for (int off = kWave / 2; off > 0; off >>= 1) {
+                float ok = __shfl_xor(key, off);
+                uint32_t oi = (uint32_t)__shfl_xor((int)ki, off);
+                if (ok < key || (ok == key && oi < ki)) {
+                    key = ok;
+                    ki = oi;
+                }
+            }
+            if (key < best.B || (key == best.B && best.Bi != kNone && ki < best.Bi)) {
+                best.B = key;
+                best.Bi = ki;
+            }
+            break;
+        }
+        const bool keep = act && !(s > best.B || (s == best.B && (best.Bi == kNone || myidx > best.Bi)));
+        const uint64_t km = __ballot(keep);
+        const uint32_t pos = (uint32_t)__popcll(km & lanemask_lt(lane));
+        wave_sync();
+        if (keep) {
+            L->act_xy[pos] = xy;
+            L->act_ms[pos] = ms;
+            L->act_idx[pos] = myidx;
+            L->act_sum[pos] = s;
+        }
+        n_act = (uint32_t)__popcll(km);
+        wave_sync();
+    }
+    wave_sync();
+}
+
+__device__ __forceinline__ void phase_clock(const BatchArgs& a, uint32_t item, int phase, int lane) {
+    if (a.dbg_clk && lane == 0) a.dbg_clk[(size_t)item * 16 + phase] = wall_clock64();
+}
+__device__ __forceinline__ void phase_value(const BatchArgs& a, uint32_t item, int slot, uint64_t v, int lane) {
+    if (a.dbg_clk && lane == 0) a.dbg_clk[(size_t)item * 16 + slot] = v;
+}
+
+// Resolves work item `item` of the batch starting at item0 with one
+// wavefront, against the state frozen at the batch start (n_res resolved
+// pixels). Writes result[item - item0].
+template <bool GUIDED>
+__device__ void resolve_item(const BatchArgs& a, const MapDesc* maps, uint32_t item, uint32_t item0, uint32_t n_res,
+                             WaveLds* L, const float* lut, const float* glut, int lane) {
     const int slot = lane % kLutCopies;
-    const uint32_t item = a.item0 + blockIdx.x * kWavesPerWG + wave;
-    if (item >= a.item1) return;
-    WaveLds* L = wls + wave;
-    const uint32_t rel = item - a.item0;
+    const uint32_t rel = item - item0;
+    phase_clock(a, item, 0, lane);
+    // the first 64 random candidates, loaded ahead of the kNN
+    uint2 rpre = make_uint2(0, 0);
+    if (!a.ext_cands && !a.knn_only && (uint32_t)lane < a.m)
+        rpre = a.rcands[(size_t)(item - a.rc_base) * a.m + lane];
     const uint32_t flat = a.work[item];
     const int x = (int)(flat % a.W), y = (int)(flat / a.W);
     const uint32_t k = a.k;
@@ -260,15 +523,25 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve(BatchArgs a) {
         if (lane < (int)kf) L->nd[lane] = a.ext_nbrs[(size_t)rel * k + lane];
     } else {
         uint32_t found = 0;
-        bool use_sel = a.n_res <= kBruteMax;
-        if (!use_sel) {
+        bool use_sel = n_res <= kBruteMax;
+        // the ring table pays off only when it likely holds k resolved pixels
+        // the offset table pays off when about k / density <= 256 offsets reach
+        // k resolved pixels; sparser masks use the tile ring search
+        const bool sparse = (uint64_t)n_res * 256ull < (uint64_t)k * a.W * a.H;
+        if (!use_sel && !sparse) {
             found = knn_table(a, L, x, y, k, lane);
-            use_sel = found < k && found < a.n_res;
+            use_sel = found < k && found < n_res;
+        } else if (!use_sel) {
+            use_sel = true;
         }
         if (use_sel) {
             Sel s{0u, ~0ull};
-            if (a.n_res <= kBruteMax) knn_list(a, L, x, y, k, lane, s);
-            else knn_rings(a, L, x, y, k, lane, s);
+            uint32_t iters = 0;
+            if (n_res <= kBruteMax) knn_list(a, L, n_res, x, y, k, lane, s);
+            else knn_rings(a, L, x, y, k, lane, s, iters);
+            phase_clock(a, item, 7, lane);
+            phase_value(a, item, 8, iters, lane);
+            phase_value(a, item, 9, s.count, lane);
             wave_sync();
             sel_compact(L->keys, s, k, lane);
             found = s.count;
@@ -277,6 +550,7 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve(BatchArgs a) {
         kf = found;
     }
     wave_sync();
+    phase_clock(a, item, 1, lane);
     if (a.dbg_nbrs) {
         if (lane < (int)kf) a.dbg_nbrs[(size_t)rel * k + lane] = L->nd[lane];
         if (lane == 0) a.dbg_nbr_count[rel] = kf;
@@ -289,9 +563,9 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve(BatchArgs a) {
             Pcg32 g0 = Pcg32::seed_from_u64(a.stage_seed);
             uint32_t rm = (uint32_t)range_u64(g0, 0, a.n_maps);
             Pcg32 g1 = Pcg32::seed_from_u64(a.stage_seed);
-            uint32_t rx = range_u32(g1, 0, a.maps[rm].w);
+            uint32_t rx = range_u32(g1, 0, maps[rm].w);
             Pcg32 g2 = Pcg32::seed_from_u64(a.stage_seed);
-            uint32_t ry = range_u32(g2, 0, a.maps[rm].h);
+            uint32_t ry = range_u32(g2, 0, maps[rm].h);
             a.result[rel] = make_uint4(rx | (ry << 16), rm | (rm << 16) | kFallbackBit, flat, __float_as_uint(0.0f));
             if (a.item_score) a.item_score[item] = -1.0f;
             if (a.dbg_best) a.dbg_best[rel] = 0xffffffffu;
@@ -317,7 +591,7 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve(BatchArgs a) {
             int cx = (int)(st.y & 0xffffu) - packed_dx(o);
             int cy = (int)(st.y >> 16) - packed_dy(o);
             uint32_t pmap = st.z >> 16;
-            coh_valid = pmap < a.n_maps && coord_ok(a.maps[pmap], cx, cy);
+            coh_valid = pmap < a.n_maps && coord_ok(maps[pmap], cx, cy);
             coh = make_uint4((uint32_t)cx | ((uint32_t)cy << 16), pmap, st.w, 0u);
         }
     }
@@ -339,12 +613,77 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve(BatchArgs a) {
     wave_sync();
     if (lane < (int)kf) L->w[lane] = (float)exp(-1.0 * (L->dist[lane] / L->avg));
     wave_sync();
+    phase_clock(a, item, 2, lane);
 
-    // ---- 3. candidate costs, one lane per candidate ----
+    // ---- 3. candidate costs ----
     const uint32_t n_c = a.ext_cands ? a.ext_cand_count[rel] : n_coh + a.m;
     float best = FLT_MAX;
     uint32_t bidx = 0;
-    for (uint32_t base = 0; base < n_c; base += kWave) {
+    const bool exact = a.exact || a.ext_full_cost || a.lut_nan || !(L->avg > 0.0);
+    if (!exact) {
+        // pruned path: unique coherence candidates first (they usually hold
+        // the winner), then the random ones 64 at a time
+        Best bst{FLT_MAX, kNone};
+        uint32_t n_k0 = n_coh;
+        if (a.ext_cands) {
+            bool k0 = false;
+            if (lane < (int)n_c) k0 = (a.ext_cands[(size_t)rel * a.ext_cmax + lane].y >> 31) == 0;
+            n_k0 = (uint32_t)__popcll(__ballot(k0));
+        }
+        // group 0: unique coherence candidates; groups 1..: random ones, 64 each
+        const uint32_t n_groups = 1 + (n_c > n_k0 ? (n_c - n_k0 + kWave - 1) / kWave : 0);
+        for (uint32_t g = 0; g < n_groups; ++g) {
+            uint32_t n_g;
+            if (g == 0) {
+                uint32_t xy = 0, ms = 0;
+                const bool v = lane < (int)n_k0;
+                if (v) cand_get(a, L, rel, item, n_coh, (uint32_t)lane, xy, ms);
+                // keep the first of equal (coordinate, map) coherence candidates:
+                // equal patterns give equal costs and the first index wins ties
+                uint64_t pending = __ballot(v), uniq = 0;
+                while (pending) {
+                    int l = __ffsll((long long)pending) - 1;
+                    uniq |= 1ull << l;
+                    uint32_t kxy = (uint32_t)__shfl((int)xy, l), kms = (uint32_t)__shfl((int)ms, l);
+                    pending &= ~__ballot(v && xy == kxy && ms == kms);
+                }
+                const bool u = (uniq >> lane) & 1ull;
+                const uint32_t pos = (uint32_t)__popcll(uniq & lanemask_lt(lane));
+                if (u) {
+                    L->act_xy[pos] = xy;
+                    L->act_ms[pos] = ms;
+                    L->act_idx[pos] = (uint32_t)lane;
+                }
+                n_g = (uint32_t)__popcll(uniq);
+                phase_clock(a, item, 3, lane);
+            } else {
+                const uint32_t g0 = n_k0 + (g - 1) * kWave;
+                const uint32_t ci = g0 + lane;
+                n_g = min((uint32_t)kWave, n_c - g0);
+                if (ci < n_c) {
+                    uint32_t xy, ms;
+                    if (!a.ext_cands && g0 == n_coh) {
+                        xy = rpre.x;
+                        ms = rpre.y | 0x80000000u;
+                    } else {
+                        cand_get(a, L, rel, item, n_coh, ci, xy, ms);
+                    }
+                    L->act_xy[lane] = xy;
+                    L->act_ms[lane] = ms;
+                    L->act_idx[lane] = ci;
+                }
+            }
+            wave_sync();
+            eval_group<GUIDED>(maps, L, lut, glut, slot, lane, kf, n_g, bst);
+            if (g == 0) phase_clock(a, item, 4, lane);
+        }
+        phase_clock(a, item, 5, lane);
+        if (bst.Bi != kNone) {
+            best = bst.B;
+            bidx = bst.Bi;
+        }
+    }
+    for (uint32_t base = 0; exact && base < n_c; base += kWave) {
         const uint32_t c = base + lane;
         const bool v = c < n_c;
         float F = 0.0f, X = -INFINITY;
@@ -367,7 +706,7 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve(BatchArgs a) {
                 cmap = r.y;
                 sgn = -1;
             }
-            const MapDesc md = a.maps[cmap];
+            const MapDesc md = maps[cmap];
             const int cx = (int)(cxy & 0xffffu), cy = (int)(cxy >> 16);
             float score = 0.0f, lastgood = -INFINITY;
             for (uint32_t j = 0; j < kf; ++j) {
@@ -441,11 +780,104 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve(BatchArgs a) {
             uint2 r = a.rcands[(size_t)(item - a.rc_base) * a.m + (bidx - n_coh)];
             cxy = r.x;
             cmap = r.y;
-            patch = (cxy >> 16) * a.maps[cmap].w + (cxy & 0xffffu);
+            patch = (cxy >> 16) * maps[cmap].w + (cxy & 0xffffu);
         }
         a.result[rel] = make_uint4(cxy, cmap | (cmap << 16), patch, __float_as_uint(best));
         if (a.item_score) a.item_score[item] = best;
         if (a.dbg_best) a.dbg_best[rel] = bidx;
+    }
+    phase_clock(a, item, 6, lane);
+}
+
+
+// Generator::update (ms.rs:334-377) for one item: colour from the stage
+// level, source coordinate, ids; new pixels also enter the resolved mask.
+__device__ __forceinline__ void commit_one(uint32_t W, uint32_t tiles_x, const MapDesc* maps, const uint32_t* work,
+                                           const uint4* result, uint32_t item, uint32_t item0, uint32_t first_new,
+                                           uint4* state, uint64_t* mask, float* score) {
+    uint32_t p = work[item];
+    uint4 r = result[item - item0];
+    uint32_t cmap = r.y & 0xffffu;
+    const MapDesc md = maps[cmap];
+    uint32_t sx = r.x & 0xffffu, sy = r.x >> 16;
+    uint32_t color = md.px[sy * md.w + sx];
+    state[p] = make_uint4(color, r.x, r.y & ~kFallbackBit, r.z);
+    if (item >= first_new) {
+        uint32_t px = p % W, py = p / W;
+        atomicOr((unsigned long long*)&mask[(py >> 3) * tiles_x + (px >> 3)], 1ull << (((py & 7u) << 3) | (px & 7u)));
+        score[p] = __uint_as_float(r.w);
+    }
+}
+
+constexpr uint32_t kMapCache = 16;  // map descriptors kept in LDS
+
+template <bool GUIDED>
+__device__ __forceinline__ WaveLds* setup_lds(const BatchArgs& a, unsigned char* smem, float*& lut, float*& glut,
+                                              MapDesc* smaps, const MapDesc*& maps) {
+    lut = reinterpret_cast<float*>(smem);
+    glut = lut + 256 * kLutCopies;
+    for (int t = threadIdx.x; t < 256 * kLutCopies; t += blockDim.x) {
+        lut[t] = a.lut[t / kLutCopies];
+        if (GUIDED) glut[t] = a.lut[256 + t / kLutCopies];
+    }
+    maps = a.maps;
+    if (a.maps && a.n_maps <= kMapCache) {
+        if (threadIdx.x < a.n_maps) smaps[threadIdx.x] = a.maps[threadIdx.x];
+        maps = smaps;
+    }
+    __syncthreads();
+    return reinterpret_cast<WaveLds*>(smem + (GUIDED ? 2 : 1) * 256 * kLutCopies * sizeof(float));
+}
+
+// One batch, one wavefront per item. With a.fused_commit the workgroup that
+// finishes last commits the whole batch (the others have written their
+// results and exited), saving the commit launch on small batches.
+template <bool GUIDED>
+__global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve(BatchArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ uint32_t is_last;
+    __shared__ MapDesc smaps[kMapCache];
+    float *lut, *glut;
+    const MapDesc* maps;
+    WaveLds* wls = setup_lds<GUIDED>(a, smem, lut, glut, smaps, maps);
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    const uint32_t item = a.item0 + blockIdx.x * kWavesPerWG + wave;
+    if (item < a.item1) resolve_item<GUIDED>(a, maps, item, a.item0, a.n_res, wls + wave, lut, glut, lane);
+    if (!a.fused_commit) return;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) is_last = atomicAdd(a.done_counter, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    for (uint32_t i = a.item0 + threadIdx.x; i < a.item1; i += blockDim.x)
+        commit_one(a.W, a.tiles_x, maps, a.work, a.result, i, a.item0, a.first_new, a.state_w, a.mask_w, a.score_w);
+    if (threadIdx.x == 0) *a.done_counter = 0u;
+}
+
+// A run of small consecutive batches (at most kSmallWaves items each) in one
+// workgroup: resolve, barrier, commit, barrier, next batch. Batch b covers
+// items [bounds[b], bounds[b+1]) with nres[b] resolved pixels at its start.
+template <bool GUIDED>
+__global__ __launch_bounds__(kWave* kSmallWaves) void k_resolve_small(BatchArgs a, const uint32_t* bounds,
+                                                                      const uint32_t* nres, uint32_t n_batches) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ MapDesc smaps[kMapCache];
+    float *lut, *glut;
+    const MapDesc* maps;
+    WaveLds* wls = setup_lds<GUIDED>(a, smem, lut, glut, smaps, maps);
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    for (uint32_t b = 0; b < n_batches; ++b) {
+        const uint32_t i0 = bounds[b], i1 = bounds[b + 1];
+        const uint32_t item = i0 + (uint32_t)wave;
+        if (item < i1) resolve_item<GUIDED>(a, maps, item, i0, nres[b], wls + wave, lut, glut, lane);
+        __syncthreads();
+        const uint32_t ci = i0 + threadIdx.x;
+        if (ci < i1) commit_one(a.W, a.tiles_x, maps, a.work, a.result, ci, i0, a.first_new, a.state_w, a.mask_w,
+                                a.score_w);
+        __syncthreads();
     }
 }
 
@@ -476,19 +908,7 @@ __global__ void k_rand(RandArgs a) {
 __global__ void k_commit(CommitArgs a) {
     uint32_t item = a.item0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= a.item1) return;
-    uint32_t p = a.work[item];
-    uint4 r = a.result[item - a.item0];
-    uint32_t cmap = r.y & 0xffffu;
-    const MapDesc md = a.maps[cmap];
-    uint32_t sx = r.x & 0xffffu, sy = r.x >> 16;
-    uint32_t color = md.px[sy * md.w + sx];
-    a.state[p] = make_uint4(color, r.x, r.y & ~kFallbackBit, r.z);
-    if (item >= a.first_new) {
-        uint32_t px = p % a.W, py = p / a.W;
-        atomicOr((unsigned long long*)&a.mask[(py >> 3) * a.tiles_x + (px >> 3)],
-                 1ull << (((py & 7u) << 3) | (px & 7u)));
-        a.score[p] = __uint_as_float(r.w);
-    }
+    commit_one(a.W, a.tiles_x, a.maps, a.work, a.result, item, a.item0, a.first_new, a.state, a.mask, a.score);
 }
 
 // next_pyramid_level (ms.rs:687-700): recolour every resolved pixel.
@@ -578,8 +998,14 @@ hipError_t configure_kernels() {
     hipError_t e = hipFuncSetAttribute((const void*)k_resolve<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)resolve_lds_bytes(true));
     if (e != hipSuccess) return e;
-    return hipFuncSetAttribute((const void*)k_resolve<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)resolve_lds_bytes(false));
+    e = hipFuncSetAttribute((const void*)k_resolve<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)resolve_lds_bytes(false));
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)k_resolve_small<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)resolve_small_lds_bytes(true));
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void*)k_resolve_small<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)resolve_small_lds_bytes(false));
 }
 
 hipError_t launch_stats(const StatsArgs& a, hipStream_t s) {
@@ -592,6 +1018,26 @@ hipError_t launch_stats(const StatsArgs& a, hipStream_t s) {
 
 size_t resolve_lds_bytes(bool guided) {
     return (guided ? 2 : 1) * 256 * kLutCopies * sizeof(float) + kWavesPerWG * sizeof(WaveLds);
+}
+
+size_t resolve_small_lds_bytes(bool guided) {
+    return (guided ? 2 : 1) * 256 * kLutCopies * sizeof(float) + kSmallWaves * sizeof(WaveLds);
+}
+
+uint32_t small_batch_max() { return kSmallWaves; }
+
+hipError_t launch_resolve_small(const BatchArgs& a, const uint32_t* bounds, const uint32_t* nres,
+                                uint32_t n_batches, bool guided, hipStream_t s) {
+    if (n_batches == 0) return hipSuccess;
+    size_t lds = resolve_small_lds_bytes(guided);
+    if (guided) {
+        hipLaunchKernelGGL(k_resolve_small<true>, dim3(1), dim3(kWave * kSmallWaves), lds, s, a, bounds, nres,
+                           n_batches);
+    } else {
+        hipLaunchKernelGGL(k_resolve_small<false>, dim3(1), dim3(kWave * kSmallWaves), lds, s, a, bounds, nres,
+                           n_batches);
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_resolve(const BatchArgs& a, bool guided, hipStream_t s) {

@@ -276,6 +276,10 @@ struct tsg_job {
     DBuf<uint32_t> overflow;
     DBuf<double> st_sum;
     DBuf<unsigned long long> st_cnt;
+    DBuf<uint32_t> done_counter;
+    DBuf<uint32_t> small_tab;  // batch bounds and resolved counts of small runs
+    size_t small_used = 0;
+    std::vector<std::vector<uint32_t>> small_keep;
     // host bookkeeping
     std::vector<uint32_t> unresolved;
     size_t n_res = 0, locked = 0, N0 = 0;
@@ -288,6 +292,8 @@ struct tsg_job {
     uint64_t n_batches = 0;
     // optional per-kernel device timing (events on the job's stream)
     bool profile = false;
+    int profile_level = 0;
+    DBuf<unsigned long long> clk;  // per-item phase clocks of the first stage (level 2)
     struct Timed {
         int kind;
         uint64_t items;
@@ -438,6 +444,7 @@ uint64_t batch_end(const tsg_job* j, uint64_t i, uint64_t redo, uint64_t S_eff) 
 }
 
 enum { K_RESOLVE = 0, K_COMMIT = 1, K_RAND = 2, K_RECOLOR = 3, K_STATS = 4 };
+constexpr uint64_t kFuseMax = 2048;  // batches up to this size commit inside the resolve launch
 
 hipError_t next_event(tsg_job* j, hipEvent_t* e) {
     if (j->ev_next == j->ev_pool.size()) {
@@ -468,6 +475,7 @@ hipError_t timed_launch(tsg_job* j, int kind, uint64_t items, F&& launch) {
 int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
     hipStream_t s = j->ctx->stream;
     if (!j->state_ready) return fail(TSG_ERR_INVALID_ARGUMENT, "job state was not initialised");
+    int rc = TSG_OK;
     j->timed.clear();
     j->ev_next = 0;
     for (int q = 0; q < 5; ++q) j->kern_ms[q] = 0.0, j->kern_items[q] = 0, j->kern_launches[q] = 0;
@@ -480,7 +488,25 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
     HIP_TRY(hipMemsetAsync(j->overflow.p, 0, 4, s));
     j->picks_keep.assign(P + 1, {});
     j->lut_keep.assign(P + 1, std::vector<float>(512));
+    j->small_keep.clear();
+    j->small_used = 0;
     j->n_batches = 0;
+    HIP_TRY(hipMemsetAsync(j->done_counter.p, 0, 4, s));
+    // Table arena of the small-batch runs: a run of nb batches takes 2 nb + 1
+    // entries. Batches of at most 16 items occur only while
+    // (resolved - locked) >> shift <= 16, i.e. for at most 17 << shift
+    // batches per segment, or for every batch when px_per_batch <= 16.
+    {
+        uint64_t items = 0;
+        for (int t = 0; t <= P; ++t) items += stage_size(j, P - t);
+        uint64_t bound = items;
+        if (j->sched.px_per_batch == 0 || j->sched.px_per_batch > small_batch_max()) {
+            uint32_t sh = std::max(j->sched.redo_shift, j->sched.new_shift);
+            uint64_t per_seg = sh >= 40 ? items : (uint64_t)(small_batch_max() + 1) << sh;
+            bound = std::min<uint64_t>(items, 2ull * (P + 1) * per_seg);
+        }
+        HIP_TRY(j->small_tab.reserve(3 * bound + 4 * (P + 1) + 16));
+    }
 
     uint64_t overall_total = 0;
     for (int t = 0; t <= P; ++t) overall_total += stage_size(j, P - t);
@@ -510,6 +536,8 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
             alpha_s = (b * b) * b;
         }
         stage_luts(j->prm.cauchy_dispersion, j->guided, alpha_s, j->lut_keep[t].data());
+        uint32_t lut_nan = 0;
+        for (float v : j->lut_keep[t]) lut_nan |= (uint32_t)std::isnan(v);
         HIP_TRY(hipMemcpyAsync(j->lut.p, j->lut_keep[t].data(), 512 * 4, hipMemcpyHostToDevice, s));
         uint64_t S_eff = S;
         if (S > redo) {
@@ -522,50 +550,100 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
         const uint64_t redo_eff = std::min(redo, S_eff);
         const uint32_t* work = j->rlist.p + j->locked;
         uint64_t rc_lo = 0, rc_hi = 0;
+        // random candidates must cover items [lo, hi)
+        auto ensure_rc = [&](uint64_t lo, uint64_t hi) -> int {
+            if (lo >= rc_lo && hi <= rc_hi) return TSG_OK;
+            uint64_t cap_items = std::max<uint64_t>(hi - lo, (uint64_t)(32u << 20) / (8ull * std::max(1u, m)));
+            rc_lo = lo;
+            rc_hi = std::min<uint64_t>(S_eff, lo + cap_items);
+            HIP_TRY(j->rc.reserve((rc_hi - rc_lo) * m));
+            RandArgs ra{maps, nf, pss, (uint32_t)rc_lo, (uint32_t)rc_hi, m, j->rc.p, j->overflow.p};
+            HIP_TRY(timed_launch(j, K_RAND, rc_hi - rc_lo, [&] { return launch_rand(ra, s); }));
+            return TSG_OK;
+        };
+        BatchArgs a{};
+        a.W = j->W;
+        a.H = j->H;
+        a.tiles_x = j->tiles_x;
+        a.tiles_y = j->tiles_y;
+        a.mask = j->mask.p;
+        a.state = j->state.p;
+        a.tg = j->guided ? j->tg.p + (size_t)level * j->N : nullptr;
+        a.maps = maps;
+        a.n_maps = nf;
+        a.work = work;
+        a.rlist = j->rlist.p;
+        a.m = m;
+        a.offtab = j->ctx->offtab.p;
+        a.offtab_n = j->ctx->offtab_n;
+        a.lut = j->lut.p;
+        a.stage_seed = pss;
+        a.k = j->prm.nearest_neighbors;
+        a.item_score = j->item_score.p;
+        a.lut_nan = lut_nan;
+        a.first_new = (uint32_t)redo_eff;
+        a.done_counter = j->done_counter.p;
+        a.dbg_clk = (j->profile_level >= 2 && t == 0) ? j->clk.p : nullptr;
+        a.state_w = j->state.p;
+        a.mask_w = j->mask.p;
+        a.score_w = j->score.p;
+        // consecutive small batches go to one single-workgroup launch
+        std::vector<uint32_t> sb_bounds, sb_nres;
+        auto flush_small = [&]() -> int {
+            if (sb_nres.empty()) return TSG_OK;
+            int rc2 = ensure_rc(sb_bounds.front(), sb_bounds.back());
+            if (rc2) return rc2;
+            j->small_keep.emplace_back(sb_bounds);
+            std::vector<uint32_t>& tab = j->small_keep.back();
+            tab.insert(tab.end(), sb_nres.begin(), sb_nres.end());
+            const size_t nb = sb_nres.size();
+            HIP_TRY(j->result.reserve(small_batch_max()));
+            // the table lives in a per-run device arena: bounds (nb+1) then nres (nb)
+            uint32_t* dt = j->small_tab.p + j->small_used;
+            if (j->small_used + tab.size() > j->small_tab.n) return fail(TSG_ERR_DEVICE, "small-batch table overflow");
+            j->small_used += tab.size();
+            HIP_TRY(hipMemcpyAsync(dt, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, s));
+            BatchArgs sa2 = a;
+            sa2.rcands = j->rc.p;
+            sa2.rc_base = (uint32_t)rc_lo;
+            sa2.result = j->result.p;
+            HIP_TRY(timed_launch(j, K_RESOLVE, sb_bounds.back() - sb_bounds.front(), [&] {
+                return launch_resolve_small(sa2, dt, dt + nb + 1, (uint32_t)nb, j->guided, s);
+            }));
+            sb_bounds.clear();
+            sb_nres.clear();
+            return TSG_OK;
+        };
         for (uint64_t i = 0; i < S_eff;) {
             uint64_t e = batch_end(j, i, redo, S_eff);
-            if (e > rc_hi) {
-                uint64_t cap_items = std::max<uint64_t>(e - i, (uint64_t)(32u << 20) / (8ull * std::max(1u, m)));
-                rc_lo = i;
-                rc_hi = std::min<uint64_t>(S_eff, i + cap_items);
-                HIP_TRY(j->rc.reserve((rc_hi - rc_lo) * m));
-                RandArgs ra{maps, nf, pss, (uint32_t)rc_lo, (uint32_t)rc_hi, m, j->rc.p, j->overflow.p};
-                HIP_TRY(timed_launch(j, K_RAND, rc_hi - rc_lo, [&] { return launch_rand(ra, s); }));
+            const uint64_t n_new = e > redo_eff ? e - std::max(i, redo_eff) : 0;
+            if (e - i <= small_batch_max()) {
+                if (sb_bounds.empty()) sb_bounds.push_back((uint32_t)i);
+                sb_bounds.push_back((uint32_t)e);
+                sb_nres.push_back((uint32_t)j->n_res);
+            } else {
+                if ((rc = flush_small())) return rc;
+                if ((rc = ensure_rc(i, e))) return rc;
+                HIP_TRY(j->result.reserve(e - i));
+                a.n_res = (uint32_t)j->n_res;
+                a.rcands = j->rc.p;
+                a.rc_base = (uint32_t)rc_lo;
+                a.item0 = (uint32_t)i;
+                a.item1 = (uint32_t)e;
+                a.result = j->result.p;
+                a.fused_commit = (e - i) <= kFuseMax;
+                HIP_TRY(timed_launch(j, K_RESOLVE, e - i, [&] { return launch_resolve(a, j->guided, s); }));
+                if (!a.fused_commit) {
+                    CommitArgs ca{j->W, j->tiles_x, maps, work, j->result.p, (uint32_t)i, (uint32_t)e,
+                                  (uint32_t)redo_eff, j->state.p, j->mask.p, j->score.p};
+                    HIP_TRY(timed_launch(j, K_COMMIT, e - i, [&] { return launch_commit(ca, s); }));
+                }
             }
-            HIP_TRY(j->result.reserve(e - i));
-            BatchArgs a{};
-            a.W = j->W;
-            a.H = j->H;
-            a.tiles_x = j->tiles_x;
-            a.tiles_y = j->tiles_y;
-            a.mask = j->mask.p;
-            a.state = j->state.p;
-            a.tg = j->guided ? j->tg.p + (size_t)level * j->N : nullptr;
-            a.maps = maps;
-            a.n_maps = nf;
-            a.work = work;
-            a.rlist = j->rlist.p;
-            a.n_res = (uint32_t)j->n_res;
-            a.rcands = j->rc.p;
-            a.rc_base = (uint32_t)rc_lo;
-            a.m = m;
-            a.offtab = j->ctx->offtab.p;
-            a.offtab_n = j->ctx->offtab_n;
-            a.lut = j->lut.p;
-            a.item0 = (uint32_t)i;
-            a.item1 = (uint32_t)e;
-            a.stage_seed = pss;
-            a.k = j->prm.nearest_neighbors;
-            a.result = j->result.p;
-            a.item_score = j->item_score.p;
-            HIP_TRY(timed_launch(j, K_RESOLVE, e - i, [&] { return launch_resolve(a, j->guided, s); }));
-            CommitArgs ca{j->W, j->tiles_x, maps, work, j->result.p, (uint32_t)i, (uint32_t)e, (uint32_t)redo_eff,
-                          j->state.p, j->mask.p, j->score.p};
-            HIP_TRY(timed_launch(j, K_COMMIT, e - i, [&] { return launch_commit(ca, s); }));
-            if (e > redo_eff) j->n_res += e - std::max(i, redo_eff);
+            j->n_res += n_new;
             j->n_batches += 1;
             i = e;
             if (cb) {
+                if ((rc = flush_small())) return rc;
                 int pct = (int)std::lround((double)(overall_cur + i) / (double)std::max<uint64_t>(overall_total, 1) *
                                            100.0);
                 if (pct != last_pct) {
@@ -579,6 +657,7 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
                 }
             }
         }
+        if ((rc = flush_small())) return rc;
         StatsArgs sa{j->item_score.p, (uint32_t)S_eff, j->st_sum.p + t, j->st_cnt.p + t};
         HIP_TRY(timed_launch(j, K_STATS, S_eff, [&] { return launch_stats(sa, s); }));
         j->st_items[t] = S_eff;
@@ -828,6 +907,8 @@ int tsg_job_create(tsg_ctx* ctx, const tsg_params* prm, uint32_t n_examples, con
     HIP_TRY(j->overflow.alloc(1));
     HIP_TRY(j->st_sum.alloc(P + 1));
     HIP_TRY(j->st_cnt.alloc(P + 1));
+    HIP_TRY(j->done_counter.alloc(1));
+    HIP_TRY(hipMemset(j->done_counter.p, 0, 4));
     // Generator::new / new_from_inpaint and the random init are content-
     // independent of the stages: upload the initial state now
     if ((rc = init_state(j.get()))) return rc;
@@ -906,9 +987,23 @@ int tsg_job_timing(tsg_job* job, double* run_ms, uint64_t* n_batches) {
     return TSG_OK;
 }
 
-int tsg_job_set_profiling(tsg_job* job, int enable) {
+int tsg_job_set_profiling(tsg_job* job, int level) {
     if (!job) return fail(TSG_ERR_INVALID_ARGUMENT, "null job");
-    job->profile = enable != 0;
+    job->profile = level != 0;
+    job->profile_level = level;
+    if (level >= 2) {
+        HIP_TRY(hipSetDevice(job->ctx->device));
+        HIP_TRY(job->clk.reserve(job->N * 16));
+        HIP_TRY(hipMemset(job->clk.p, 0, job->N * 16 * 8));
+    }
+    return TSG_OK;
+}
+
+int tsg_job_phase_clocks(tsg_job* job, uint64_t* out, uint64_t n_items) {
+    if (!job || !out) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    if (job->profile_level < 2) return fail(TSG_ERR_INVALID_ARGUMENT, "phase clocks need profiling level 2");
+    n_items = std::min<uint64_t>(n_items, job->N);
+    HIP_TRY(hipMemcpy(out, job->clk.p, n_items * 16 * 8, hipMemcpyDeviceToHost));
     return TSG_OK;
 }
 
@@ -1067,7 +1162,7 @@ int tsg_score_batch(tsg_ctx* ctx, uint32_t w, uint32_t h, const uint8_t* color, 
                     const tsg_image* target_guide, const float* lut_color, const float* lut_guide, uint32_t k,
                     uint32_t cmax, uint32_t n, const uint32_t* flat, const uint32_t* nbr_count, const int32_t* nbrs,
                     const uint32_t* cand_count, const int32_t* cands, float* full_cost, uint32_t* best,
-                    float* best_score) {
+                    float* best_score, uint32_t* best_pruned, float* best_score_pruned) {
     if (!ctx || !color || !coord || !ids || !maps || !lut_color || !flat || !nbr_count || !nbrs || !cand_count ||
         !cands || !full_cost || !best || !best_score)
         return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
@@ -1078,11 +1173,18 @@ int tsg_score_batch(tsg_ctx* ctx, uint32_t w, uint32_t h, const uint8_t* color, 
     for (uint32_t i = 0; i < n; ++i) {
         if (flat[i] >= w * h || nbr_count[i] > k || cand_count[i] > cmax || nbr_count[i] == 0 || cand_count[i] == 0)
             return fail(TSG_ERR_INVALID_ARGUMENT, "bad item");
+        int last_kind = 0;
+        uint32_t n_k0 = 0;
         for (uint32_t c = 0; c < cand_count[i]; ++c) {
             const int32_t* e = cands + ((size_t)i * cmax + c) * 6;
             if ((uint32_t)e[2] >= n_maps || e[0] < 0 || e[1] < 0 || e[0] > 65535 || e[1] > 65535)
                 return fail(TSG_ERR_INVALID_ARGUMENT, "bad candidate");
+            int kind = e[3] != 0;
+            if (kind < last_kind) return fail(TSG_ERR_INVALID_ARGUMENT, "coherence candidates must come first");
+            last_kind = kind;
+            n_k0 += kind == 0;
         }
+        if (n_k0 > (uint32_t)kMaxK) return fail(TSG_ERR_INVALID_ARGUMENT, "more than 64 coherence candidates");
         for (uint32_t jn = 0; jn < nbr_count[i]; ++jn) {
             int nx = nbrs[((size_t)i * k + jn) * 2], ny = nbrs[((size_t)i * k + jn) * 2 + 1];
             if (nx < 0 || ny < 0 || (uint32_t)nx >= w || (uint32_t)ny >= h)
@@ -1184,6 +1286,9 @@ int tsg_score_batch(tsg_ctx* ctx, uint32_t w, uint32_t h, const uint8_t* color, 
     a.ext_cmax = cmax;
     a.ext_full_cost = dfc.p;
     a.dbg_best = dbest.p;
+    bool lut_nan = false;
+    for (float v : lut) lut_nan |= std::isnan(v);
+    a.lut_nan = lut_nan;
     HIP_TRY(launch_resolve(a, guided, s));
     HIP_TRY(hipStreamSynchronize(s));
     std::vector<uint4> res(n);
@@ -1191,6 +1296,19 @@ int tsg_score_batch(tsg_ctx* ctx, uint32_t w, uint32_t h, const uint8_t* color, 
     HIP_TRY(hipMemcpy(full_cost, dfc.p, (size_t)n * cmax * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(best, dbest.p, n * 4, hipMemcpyDeviceToHost));
     for (uint32_t i = 0; i < n; ++i) memcpy(&best_score[i], &res[i].w, 4);
+    if (best_pruned || best_score_pruned) {
+        // production evaluation (dedup + pruning) of the same lists
+        a.ext_full_cost = nullptr;
+        HIP_TRY(launch_resolve(a, guided, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        HIP_TRY(hipMemcpy(res.data(), dres.p, n * sizeof(uint4), hipMemcpyDeviceToHost));
+        std::vector<uint32_t> bp(n);
+        HIP_TRY(hipMemcpy(bp.data(), dbest.p, n * 4, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; ++i) {
+            if (best_pruned) best_pruned[i] = bp[i];
+            if (best_score_pruned) memcpy(&best_score_pruned[i], &res[i].w, 4);
+        }
+    }
     return TSG_OK;
 }
 

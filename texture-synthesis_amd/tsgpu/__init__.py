@@ -104,12 +104,13 @@ def lib():
     L.tsg_job_timing.argtypes = [vp, vp, vp]
     L.tsg_job_set_profiling.argtypes = [vp, C.c_int]
     L.tsg_job_kernel_stats.argtypes = [vp, vp, vp, vp]
+    L.tsg_job_phase_clocks.argtypes = [vp, vp, u64]
     L.tsg_job_destroy.argtypes = [vp]
     L.tsg_pyramid_build.argtypes = [vp, P(Image), u32, vp]
     L.tsg_knn_batch.argtypes = [vp, vp, u32, u32, u32, vp, u32, vp, vp]
     L.tsg_random_candidates.argtypes = [vp, u32, P(Image), P(Image), u64, u32, u32, u32, vp]
     L.tsg_score_batch.argtypes = [vp, u32, u32, vp, vp, vp, u32, P(Image), P(Image), P(Image), vp, vp, u32, u32,
-                                  u32, vp, vp, vp, vp, vp, vp, vp, vp]
+                                  u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -316,9 +317,9 @@ class Session:
         _check(lib().tsg_job_set_schedule(self._job, C.byref(s)))
         return self
 
-    def set_profiling(self, enable: bool = True) -> "Session":
-        _check(lib().tsg_job_set_profiling(self._job, int(enable)))
-        self._profiled = enable
+    def set_profiling(self, level: int = 1) -> "Session":
+        _check(lib().tsg_job_set_profiling(self._job, int(level)))
+        self._profiled = level
         return self
 
     def run(self, progress=None) -> GeneratedImage:
@@ -359,6 +360,10 @@ class Session:
                 _check(L.tsg_job_kernel_stats(self._job, _ptr(kms), _ptr(kit), _ptr(kla)))
                 kst = {n: dict(ms=float(kms[i]), items=int(kit[i]), launches=int(kla[i]))
                        for i, n in enumerate(GeneratedImage.KERNELS)}
+                if getattr(self, "_profiled", 0) >= 2:
+                    clk = np.zeros((h * w, 16), np.uint64)
+                    _check(L.tsg_job_phase_clocks(self._job, _ptr(clk), h * w))
+                    kst["phase_clocks"] = clk
             return GeneratedImage(img, coord, ids, rf, rs, (sums[:got], cnts[:got], items[:got]), ms.value,
                                   nb.value, self._original_maps, kst)
         finally:
@@ -551,8 +556,10 @@ def score_batch(color, coord, ids, maps, lut_color, k, flat, nbr_count, nbrs, ca
     full = np.zeros((n, cmax), np.float32)
     best = np.zeros(n, np.uint32)
     bs = np.zeros(n, np.float32)
+    bp = np.zeros(n, np.uint32)
+    bsp = np.zeros(n, np.float32)
     _check(lib().tsg_score_batch(ctx._h, w, h, _ptr(color), _ptr(coord), _ptr(ids), len(maps), M.arr,
                                  G.arr if G else None, T.arr if T else None, _ptr(lc), _ptr(lg), k, cmax, n,
                                  _ptr(flat), _ptr(nbr_count), _ptr(nbrs), _ptr(cand_count), _ptr(cands),
-                                 _ptr(full), _ptr(best), _ptr(bs)))
-    return full, best, bs
+                                 _ptr(full), _ptr(best), _ptr(bs), _ptr(bp), _ptr(bsp)))
+    return full, best, bs, bp, bsp
