@@ -10,7 +10,19 @@ reference's criterion benches build outside the timer
 Prints one JSON line: Mpix/s of output, per-step latency p50/p99, the
 roofline of the dominant kernel (k_resolve) from HIP events on the
 library's stream, and the CPU oracle timed on the host's cores.
+
+Two more legs report in the same line:
+* "batched": 256 independent syntheses (examples uniform in {64..512}^2,
+  outputs uniform in {128..1024}^2, k in {20, 50}, m = 50, seeds 4000+j)
+  run together through the multi-job engine, against the same 256 run one
+  after another through Session.run; aggregate output Mpix/s of both, and
+  whether every batched result equals its sequential twin bit for bit.
+* "serving": a Poisson request loop at a fixed rate; outputs log-uniform
+  over 128^2..2048^2 from seeded 256^2 examples, 30% inpainting with 1-3
+  elliptical holes covering 20-40%; admission by device-memory footprint;
+  p50/p99 latency (arrival to result on the host) and output Mpix/s.
 """
+import hashlib
 import argparse
 import json
 import os
@@ -49,6 +61,117 @@ def cpu_baseline(ex, threads):
                        f"(oracle, same Jacobi schedule), {dt:.1f} s")
 
 
+def digest(g):
+    h = hashlib.blake2b(digest_size=16)
+    for a in (g.image, g.coord, g.ids, g.resolved_flat, g.resolved_score):
+        h.update(a.tobytes())
+    return h.hexdigest()
+
+
+def batched_specs(n, seed=4000):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ex = rng.choice([64, 128, 256, 512], n)
+    out = rng.choice([128, 256, 512, 1024], n)
+    k = rng.choice([20, 50], n)
+    return [(int(ex[j]), int(out[j]), int(k[j]), seed + j) for j in range(n)]
+
+
+def batched_leg(ts, synth, n):
+    """n syntheses one after another (Session.run) and all at once (Batch)."""
+    specs = batched_specs(n)
+    exs = [synth.example(sd, e, e) for e, _, _, sd in specs]
+
+    def builders():
+        return [ts.Session.builder().add_example(x).seed(sd).nearest_neighbors(k).random_sample_locations(M)
+                .output_size(ts.Dims.square(o)) for x, (e, o, k, sd) in zip(exs, specs)]
+
+    px = sum(o * o for _, o, _, _ in specs)
+    sess = [b.build() for b in builders()]
+    t0 = time.perf_counter()
+    seq = [digest(s.run()) for s in sess]
+    t_seq = time.perf_counter() - t0
+    del sess
+    sess = [b.build() for b in builders()]
+    batch = ts.Batch(ts.Context.default(), max_active=n)
+    t0 = time.perf_counter()
+    tickets = [batch.submit(s) for s in sess]
+    batch.run_all()
+    got = [digest(batch.take(t)) for t in tickets]
+    t_bat = time.perf_counter() - t0
+    st = batch.stats()
+    batch.close()
+    return {
+        "workload": f"{n} syntheses: examples uniform in {{64,128,256,512}}^2, outputs uniform in "
+                    f"{{128,256,512,1024}}^2, k in {{20,50}}, m=50, seeds 4000+j",
+        "output_mpix": round(px / 1e6, 3),
+        "value": round(px / t_bat / 1e6, 3), "unit": "Mpix/s",
+        "sequential_value": round(px / t_seq / 1e6, 3),
+        "speedup_vs_sequential": round(t_seq / t_bat, 2),
+        "seconds": round(t_bat, 3), "sequential_seconds": round(t_seq, 3),
+        "identical_to_alone": seq == got,
+        "rounds": st["rounds"], "launches": st["launches"],
+    }
+
+
+def serving_requests(ts, synth, n, seed=5000):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sides = np.exp(rng.uniform(np.log(128), np.log(2048), n)).astype(int)
+    inpaint = rng.random(n) < 0.3
+    reqs = []
+    for i in range(n):
+        side, sd = int(sides[i]), seed + i
+        ex = synth.example(sd, 256, 256)
+        b = ts.Session.builder().seed(sd)
+        if inpaint[i]:
+            idx = (np.arange(side) * 256) // side
+            m = synth.ellipse_mask(sd, side, side)
+            b.inpaint_example(m, ts.Example(ex[idx][:, idx]).set_sample_method(m), ts.Dims.square(side))
+        else:
+            b.add_example(ex).output_size(ts.Dims.square(side))
+        reqs.append((b, side * side, bool(inpaint[i])))
+    return reqs
+
+
+def serving_leg(ts, synth, n, qps, budget_gib):
+    from tsgpu import serve
+    import numpy as np
+    reqs = serving_requests(ts, synth, n)
+    gaps = np.random.default_rng(5999).exponential(1.0 / qps, n)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    srv = serve.Server(ts.Context.default(), hbm_budget=int(budget_gib * (1 << 30)), max_active=256,
+                       rounds_per_pump=4)
+    fin = []
+    t0 = time.perf_counter()
+    i = 0
+    while i < n or not srv.idle():
+        now = time.perf_counter() - t0
+        while i < n and arrivals[i] <= now:
+            srv.submit(reqs[i][0], arrival=t0 + arrivals[i])
+            i += 1
+        if srv.idle():
+            time.sleep(max(0.0, min(0.01, arrivals[i] - now)))
+            continue
+        fin.extend(srv.pump())
+    t_end = time.perf_counter()
+    srv.close()
+    errors = [f for f in fin if f.error is not None]
+    lat = [f.latency_s * 1e3 for f in fin if f.error is None]
+    px = sum(f.out_px for f in fin if f.error is None)
+    return {
+        "workload": f"{n} requests, Poisson {qps}/s, outputs log-uniform 128^2..2048^2 from seeded 256^2 "
+                    f"examples, {sum(r[2] for r in reqs)} inpainting (1-3 ellipses, 20-40% holes)",
+        "value": round(px / (t_end - t0) / 1e6, 3), "unit": "Mpix/s",
+        "latency_ms": {"p50": round(serve.percentile(lat, 50), 1), "p99": round(serve.percentile(lat, 99), 1),
+                       "max": round(max(lat), 1) if lat else None},
+        "offered_mpix_per_s": round(sum(r[1] for r in reqs) / 1e6 * qps / n, 3),
+        "seconds": round(t_end - t0, 3), "errors": len(errors),
+        "hbm_budget_gib": budget_gib, "peak_admitted_gib": round(srv.peak_in_use / (1 << 30), 3),
+        "memory_holds": srv.memory_holds,
+    }
+
+
 def traffic_from_profiles():
     path = os.path.join(ROOT, "profiles", "resolve_c2_traffic.json")
     if not os.path.exists(path):
@@ -64,6 +187,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batched", type=int, default=256, help="syntheses in the batched leg (0 = skip)")
+    ap.add_argument("--serve-requests", type=int, default=120, help="requests in the serving leg (0 = skip)")
+    ap.add_argument("--serve-qps", type=float, default=12.0)
+    ap.add_argument("--serve-budget-gib", type=float, default=4.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X; --gpus must be 1", file=sys.stderr)
@@ -120,6 +247,11 @@ def main():
         "kernels_ms": {k: round(v["ms"], 3) for k, v in ks.items()},
         "device_ms_per_step": round(g.run_ms, 3),
     }
+    if args.batched:
+        out["batched"] = batched_leg(ts, synth, args.batched)
+        out["batched"]["vs_plain"] = round(out["batched"]["value"] / value, 2)
+    if args.serve_requests:
+        out["serving"] = serving_leg(ts, synth, args.serve_requests, args.serve_qps, args.serve_budget_gib)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ex, args.cpu_threads)
         out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)

@@ -143,6 +143,38 @@ int tsg_job_phase_clocks(tsg_job* job, uint64_t* out, uint64_t n_items);
 int tsg_job_kernel_stats(tsg_job* job, double* ms, uint64_t* items, uint64_t* launches);
 void tsg_job_destroy(tsg_job* job);
 
+/* ---- many syntheses at once (no counterpart in the reference: serving and
+ *      batch callers that run many Session::run at the same time) ----
+ * A tsg_batch advances every admitted job in lockstep rounds: each round
+ * resolves every active job's next Jacobi batch in one launch and commits
+ * them in a second. A job's result is identical to tsg_job_run on it alone.
+ * Jobs are created with tsg_job_create and handed over with
+ * tsg_batch_submit; once finished they are taken back with tsg_batch_take
+ * (then downloaded and destroyed by the caller). */
+typedef struct tsg_batch tsg_batch;
+/* Device bytes held by a job (pyramids, state, scratch). */
+int tsg_job_footprint(const tsg_job* job, uint64_t* bytes);
+/* Device bytes a job of these dimensions holds while it runs under a
+ * tsg_batch, before it is created (for admission by memory). example_wh:
+ * n_examples (w, h) pairs; sample_kind may be NULL (all TSG_SAMPLE_ALL);
+ * guide_wh: NULL when unguided, else the example guides' (w, h) pairs. */
+int tsg_footprint_estimate(const tsg_params* p, uint32_t n_examples, const uint32_t* example_wh,
+                           const int* sample_kind, const uint32_t* guide_wh, uint32_t out_w, uint32_t out_h,
+                           uint64_t* bytes);
+/* At most max_active jobs run at a time; the rest wait in submission order. */
+int tsg_batch_create(tsg_ctx* ctx, uint32_t max_active, tsg_batch** out);
+int tsg_batch_submit(tsg_batch* b, tsg_job* job, uint64_t* ticket);
+/* Runs up to max_rounds rounds; in_flight = jobs not yet reported finished. */
+int tsg_batch_step(tsg_batch* b, uint32_t max_rounds, uint32_t* in_flight);
+/* Runs until every submitted job has finished. */
+int tsg_batch_run_all(tsg_batch* b);
+/* Finished jobs not yet taken: tickets and their run status (0 = ok). */
+int tsg_batch_poll(tsg_batch* b, uint64_t* tickets, int* status, uint32_t cap, uint32_t* n);
+int tsg_batch_take(tsg_batch* b, uint64_t ticket, tsg_job** job);
+int tsg_batch_stats(tsg_batch* b, uint64_t* rounds, uint64_t* launches, uint64_t* units);
+/* Destroys the batch and every job it still holds. */
+void tsg_batch_destroy(tsg_batch* b);
+
 /* ---- test and tooling entry points (no counterpart in the reference API;
  *      they expose single steps of the path for parity tests) ---- */
 
@@ -175,14 +207,16 @@ int tsg_random_candidates(tsg_ctx* ctx, uint32_t n_maps, const tsg_image* maps, 
  * 1=random, patch, patch map).
  * outputs: full_cost[i*cmax + c] (no early exit), best[i], best_score[i]
  * from the unpruned evaluation; best_pruned / best_score_pruned (nullable)
- * from the production evaluation (coherence dedup + exact pruning).
+ * from the production evaluation (coherence dedup + exact pruning) with
+ * `team` wavefronts per item (0 or 1, 2, 4, 8; the candidate list is split
+ * between them).
  * Coherence candidates (kind 0, at most 64) must precede random ones. */
 int tsg_score_batch(tsg_ctx* ctx, uint32_t w, uint32_t h, const uint8_t* color, const uint32_t* coord,
                     const uint32_t* ids, uint32_t n_maps, const tsg_image* maps, const tsg_image* guides,
                     const tsg_image* target_guide, const float* lut_color, const float* lut_guide, uint32_t k,
                     uint32_t cmax, uint32_t n, const uint32_t* flat, const uint32_t* nbr_count, const int32_t* nbrs,
                     const uint32_t* cand_count, const int32_t* cands, float* full_cost, uint32_t* best,
-                    float* best_score, uint32_t* best_pruned, float* best_score_pruned);
+                    float* best_score, uint32_t* best_pruned, float* best_score_pruned, uint32_t team);
 
 #ifdef __cplusplus
 }

@@ -138,7 +138,7 @@ def _frozen(examples, out, stage_t, item, n, threads=8, **kw):
 
 
 def _score_on_device(ctx, info, rgba, coord, ids, my, gl, d, examples, k, sample_kind=None, example_guides=None,
-                     target_guide=None):
+                     target_guide=None, team=1):
     lv = info["level"]
     kinds = sample_kind or [0] * len(examples)
     maps = [R.pyramid(e, 5)[lv] for e, kd in zip(examples, kinds) if kd != 1]
@@ -147,7 +147,8 @@ def _score_on_device(ctx, info, rgba, coord, ids, my, gl, d, examples, k, sample
         guides = [R.pyramid(g, 5)[lv] for g in example_guides][:len(maps)]
         tg = R.pyramid(target_guide, 5)[lv]
     return ts.score_batch(rgba, coord, ids, maps, my, k, d["flat"], d["nbr_count"], d["nbrs"], d["cand_count"],
-                          d["cands"], guides=guides, target_guide=tg, lut_guide=gl if guides else None, ctx=ctx)
+                          d["cands"], guides=guides, target_guide=tg, lut_guide=gl if guides else None, ctx=ctx,
+                          team=team)
 
 
 def _check_scores(d, full, best, bs, bp, bsp):
@@ -163,12 +164,14 @@ def _check_scores(d, full, best, bs, bp, bsp):
     assert np.array_equal(bsp[ok].view(np.uint32), d["best_score"][ok].view(np.uint32))
 
 
-@pytest.mark.parametrize("stage_t,item", [(4, 0), (4, 16384), (2, 3000)])
-def test_frozen_state_scoring_bit_exact_256(ctx, stage_t, item):
+@pytest.mark.parametrize("stage_t,item,team", [(4, 0, 1), (4, 16384, 1), (2, 3000, 1), (4, 0, 8), (2, 3000, 2),
+                                                (4, 16384, 4)])
+def test_frozen_state_scoring_bit_exact_256(ctx, stage_t, item, team):
+    # team > 1: the candidate list split between waves, winner merged
     exs = [synth.example(211, 128, 128)]
     job, info, rgba, coord, ids, my, gl, d = _frozen(exs, 256, stage_t, item, 4096, seed=211)
     assert len(d["flat"]) > 100
-    full, best, bs, bp, bsp = _score_on_device(ctx, info, rgba, coord, ids, my, gl, d, exs, 50)
+    full, best, bs, bp, bsp = _score_on_device(ctx, info, rgba, coord, ids, my, gl, d, exs, 50, team=team)
     _check_scores(d, full, best, bs, bp, bsp)
 
 
@@ -179,9 +182,10 @@ def test_frozen_state_scoring_guided_multi_example(ctx):
     job, info, rgba, coord, ids, my, gl, d = _frozen(exs, 96, 3, 500, 2048, seed=211, example_guides=gds,
                                                      target_guide=tg)
     assert info["alpha"] > 0
-    full, best, bs, bp, bsp = _score_on_device(ctx, info, rgba, coord, ids, my, gl, d, exs, 50, example_guides=gds,
-                                      target_guide=tg)
-    _check_scores(d, full, best, bs, bp, bsp)
+    for team in (1, 8):
+        full, best, bs, bp, bsp = _score_on_device(ctx, info, rgba, coord, ids, my, gl, d, exs, 50,
+                                                   example_guides=gds, target_guide=tg, team=team)
+        _check_scores(d, full, best, bs, bp, bsp)
 
 
 def test_random_candidates_bit_exact(ctx):

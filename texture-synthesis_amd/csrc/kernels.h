@@ -52,7 +52,8 @@ struct BatchArgs {
     uint4* state_w;
     uint64_t* mask_w;
     float* score_w;
-    unsigned long long* dbg_clk;  // [stage item][8] phase timestamps (profiling)
+    unsigned long long* dbg_clk;  // [stage item][16] phase timestamps (profiling)
+    uint32_t team;                // waves per item (1, 2, 4, 8; 0 = 1)
 };
 
 struct RandArgs {
@@ -96,6 +97,16 @@ hipError_t configure_kernels();
 size_t resolve_lds_bytes(bool guided);
 size_t resolve_small_lds_bytes(bool guided);
 uint32_t small_batch_max();
+// items per workgroup of k_resolve / k_resolve_multi with `team` waves per item
+uint32_t items_per_wg(uint32_t team);
+// waves per item for a batch of n items (1, 2, 4 or 8)
+uint32_t team_for(uint64_t n);
+// one round of the multi-job engine: units[u] is a job's batch, its workgroups
+// start at wg_start[u] (n_units + 1 entries), its items at item_start[u]
+hipError_t launch_resolve_multi(const BatchArgs* units, const uint32_t* wg_start, uint32_t n_units, uint32_t n_wg,
+                                bool guided, hipStream_t s);
+hipError_t launch_commit_multi(const BatchArgs* units, const uint32_t* item_start, uint32_t n_units, uint32_t total,
+                               hipStream_t s);
 // consecutive batches of at most small_batch_max() items, one workgroup
 hipError_t launch_resolve_small(const BatchArgs& a, const uint32_t* bounds, const uint32_t* nres,
                                 uint32_t n_batches, bool guided, hipStream_t s);

@@ -76,6 +76,10 @@ struct WaveLds {
     uint32_t act_idx[kWave];  // index in the reference's candidate list
     float act_sum[kWave];
     double avg;
+    // team exchange: neighbour count (leader), best (cost, index) per member
+    uint32_t t_kf;
+    float t_B;
+    uint32_t t_Bi;
 };
 
 // ---- k-smallest selection over canonical keys, one wavefront ----
@@ -499,24 +503,23 @@ __device__ __forceinline__ void phase_value(const BatchArgs& a, uint32_t item, i
     if (a.dbg_clk && lane == 0) a.dbg_clk[(size_t)item * 16 + slot] = v;
 }
 
-// Resolves work item `item` of the batch starting at item0 with one
-// wavefront, against the state frozen at the batch start (n_res resolved
-// pixels). Writes result[item - item0].
-template <bool GUIDED>
-__device__ void resolve_item(const BatchArgs& a, const MapDesc* maps, uint32_t item, uint32_t item0, uint32_t n_res,
-                             WaveLds* L, const float* lut, const float* glut, int lane) {
-    const int slot = lane % kLutCopies;
-    const uint32_t rel = item - item0;
-    phase_clock(a, item, 0, lane);
-    // the first 64 random candidates, loaded ahead of the kNN
-    uint2 rpre = make_uint2(0, 0);
-    if (!a.ext_cands && !a.knn_only && (uint32_t)lane < a.m)
-        rpre = a.rcands[(size_t)(item - a.rc_base) * a.m + lane];
-    const uint32_t flat = a.work[item];
-    const int x = (int)(flat % a.W), y = (int)(flat / a.W);
-    const uint32_t k = a.k;
+// ---- one work item, resolved by a team of T wavefronts (T = 1, 2, 4, 8, 16) ----
+//
+// The team leader (member 0) finds the k nearest resolved neighbours; every
+// member then gathers the neighbour state, weights and coherence candidates
+// itself (identical values), evaluates its own slice of the candidate list
+// with exact pruning, and leaves its best (cost, index) in LDS; the leader
+// takes the lexicographic minimum, which is the reference's winner (the
+// first candidate of least cost), and writes the result. Members of one
+// team share a workgroup; the two workgroup barriers are taken by every wave
+// of the workgroup whatever its item does, so T > 1 needs all waves of the
+// workgroup to call this together. With T = 1 there is no barrier.
 
-    // ---- 1. k nearest resolved neighbours, canonical order ----
+// Leader, step 1: kNN into L->nd; returns the count. Test hooks: host lists,
+// dumps, knn_only.
+__device__ __forceinline__ uint32_t item_knn(const BatchArgs& a, uint32_t item, uint32_t rel, int x, int y,
+                                             uint32_t n_res, WaveLds* L, int lane) {
+    const uint32_t k = a.k;
     uint32_t kf;
     if (a.ext_nbrs) {
         kf = a.ext_nbr_count[rel];
@@ -524,7 +527,6 @@ __device__ void resolve_item(const BatchArgs& a, const MapDesc* maps, uint32_t i
     } else {
         uint32_t found = 0;
         bool use_sel = n_res <= kBruteMax;
-        // the ring table pays off only when it likely holds k resolved pixels
         // the offset table pays off when about k / density <= 256 offsets reach
         // k resolved pixels; sparser masks use the tile ring search
         const bool sparse = (uint64_t)n_res * 256ull < (uint64_t)k * a.W * a.H;
@@ -550,30 +552,37 @@ __device__ void resolve_item(const BatchArgs& a, const MapDesc* maps, uint32_t i
         kf = found;
     }
     wave_sync();
-    phase_clock(a, item, 1, lane);
     if (a.dbg_nbrs) {
         if (lane < (int)kf) a.dbg_nbrs[(size_t)rel * k + lane] = L->nd[lane];
         if (lane == 0) a.dbg_nbr_count[rel] = kf;
     }
-    if (a.knn_only) return;
+    return kf;
+}
 
-    if (kf == 0) {
-        // no resolved neighbour: resolve_at_random with the stage seed (ms.rs:1003-1009)
-        if (lane == 0) {
-            Pcg32 g0 = Pcg32::seed_from_u64(a.stage_seed);
-            uint32_t rm = (uint32_t)range_u64(g0, 0, a.n_maps);
-            Pcg32 g1 = Pcg32::seed_from_u64(a.stage_seed);
-            uint32_t rx = range_u32(g1, 0, maps[rm].w);
-            Pcg32 g2 = Pcg32::seed_from_u64(a.stage_seed);
-            uint32_t ry = range_u32(g2, 0, maps[rm].h);
-            a.result[rel] = make_uint4(rx | (ry << 16), rm | (rm << 16) | kFallbackBit, flat, __float_as_uint(0.0f));
-            if (a.item_score) a.item_score[item] = -1.0f;
-            if (a.dbg_best) a.dbg_best[rel] = 0xffffffffu;
-        }
-        return;
-    }
+// Leader: no resolved neighbour, resolve_at_random with the stage seed
+// (ms.rs:1003-1009).
+__device__ __forceinline__ void item_at_random(const BatchArgs& a, const MapDesc* maps, uint32_t item, uint32_t rel,
+                                               uint32_t flat, int lane) {
+    if (lane != 0) return;
+    Pcg32 g0 = Pcg32::seed_from_u64(a.stage_seed);
+    uint32_t rm = (uint32_t)range_u64(g0, 0, a.n_maps);
+    Pcg32 g1 = Pcg32::seed_from_u64(a.stage_seed);
+    uint32_t rx = range_u32(g1, 0, maps[rm].w);
+    Pcg32 g2 = Pcg32::seed_from_u64(a.stage_seed);
+    uint32_t ry = range_u32(g2, 0, maps[rm].h);
+    a.result[rel] = make_uint4(rx | (ry << 16), rm | (rm << 16) | kFallbackBit, flat, __float_as_uint(0.0f));
+    if (a.item_score) a.item_score[item] = -1.0f;
+    if (a.dbg_best) a.dbg_best[rel] = 0xffffffffu;
+}
 
-    // ---- 2. neighbour state: own pattern, distances, coherence candidates ----
+// Every member: neighbour state (own pattern, distances, weights) into L and
+// the coherence candidates; returns n_coh. Lanes < n_k0 hold candidate
+// lane's (xy, map) in xy/ms; `uniq` marks the first of equal ones.
+template <bool GUIDED>
+__device__ __forceinline__ uint32_t item_neighbours(const BatchArgs& a, const MapDesc* maps, uint32_t item,
+                                                    uint32_t rel, int x, int y, uint32_t kf, uint32_t n_k0,
+                                                    WaveLds* L, int lane,
+                                                    uint32_t& xy, uint32_t& ms, uint64_t& uniq) {
     bool coh_valid = false;
     uint4 coh = make_uint4(0, 0, 0, 0);
     if (lane < (int)kf) {
@@ -613,77 +622,32 @@ __device__ void resolve_item(const BatchArgs& a, const MapDesc* maps, uint32_t i
     wave_sync();
     if (lane < (int)kf) L->w[lane] = (float)exp(-1.0 * (L->dist[lane] / L->avg));
     wave_sync();
-    phase_clock(a, item, 2, lane);
-
-    // ---- 3. candidate costs ----
-    const uint32_t n_c = a.ext_cands ? a.ext_cand_count[rel] : n_coh + a.m;
-    float best = FLT_MAX;
-    uint32_t bidx = 0;
-    const bool exact = a.exact || a.ext_full_cost || a.lut_nan || !(L->avg > 0.0);
-    if (!exact) {
-        // pruned path: unique coherence candidates first (they usually hold
-        // the winner), then the random ones 64 at a time
-        Best bst{FLT_MAX, kNone};
-        uint32_t n_k0 = n_coh;
-        if (a.ext_cands) {
-            bool k0 = false;
-            if (lane < (int)n_c) k0 = (a.ext_cands[(size_t)rel * a.ext_cmax + lane].y >> 31) == 0;
-            n_k0 = (uint32_t)__popcll(__ballot(k0));
-        }
-        // group 0: unique coherence candidates; groups 1..: random ones, 64 each
-        const uint32_t n_groups = 1 + (n_c > n_k0 ? (n_c - n_k0 + kWave - 1) / kWave : 0);
-        for (uint32_t g = 0; g < n_groups; ++g) {
-            uint32_t n_g;
-            if (g == 0) {
-                uint32_t xy = 0, ms = 0;
-                const bool v = lane < (int)n_k0;
-                if (v) cand_get(a, L, rel, item, n_coh, (uint32_t)lane, xy, ms);
-                // keep the first of equal (coordinate, map) coherence candidates:
-                // equal patterns give equal costs and the first index wins ties
-                uint64_t pending = __ballot(v), uniq = 0;
-                while (pending) {
-                    int l = __ffsll((long long)pending) - 1;
-                    uniq |= 1ull << l;
-                    uint32_t kxy = (uint32_t)__shfl((int)xy, l), kms = (uint32_t)__shfl((int)ms, l);
-                    pending &= ~__ballot(v && xy == kxy && ms == kms);
-                }
-                const bool u = (uniq >> lane) & 1ull;
-                const uint32_t pos = (uint32_t)__popcll(uniq & lanemask_lt(lane));
-                if (u) {
-                    L->act_xy[pos] = xy;
-                    L->act_ms[pos] = ms;
-                    L->act_idx[pos] = (uint32_t)lane;
-                }
-                n_g = (uint32_t)__popcll(uniq);
-                phase_clock(a, item, 3, lane);
-            } else {
-                const uint32_t g0 = n_k0 + (g - 1) * kWave;
-                const uint32_t ci = g0 + lane;
-                n_g = min((uint32_t)kWave, n_c - g0);
-                if (ci < n_c) {
-                    uint32_t xy, ms;
-                    if (!a.ext_cands && g0 == n_coh) {
-                        xy = rpre.x;
-                        ms = rpre.y | 0x80000000u;
-                    } else {
-                        cand_get(a, L, rel, item, n_coh, ci, xy, ms);
-                    }
-                    L->act_xy[lane] = xy;
-                    L->act_ms[lane] = ms;
-                    L->act_idx[lane] = ci;
-                }
-            }
-            wave_sync();
-            eval_group<GUIDED>(maps, L, lut, glut, slot, lane, kf, n_g, bst);
-            if (g == 0) phase_clock(a, item, 4, lane);
-        }
-        phase_clock(a, item, 5, lane);
-        if (bst.Bi != kNone) {
-            best = bst.B;
-            bidx = bst.Bi;
-        }
+    // keep the first of equal (coordinate, map) coherence candidates: equal
+    // patterns give equal costs and the first index wins ties (a host list
+    // leads with its n_k0 coherence entries)
+    const uint32_t nk = a.ext_cands ? n_k0 : n_coh;
+    xy = 0;
+    ms = 0;
+    const bool v = lane < (int)nk;
+    if (v) cand_get(a, L, rel, item, n_coh, (uint32_t)lane, xy, ms);
+    uint64_t pending = __ballot(v);
+    uniq = 0;
+    while (pending) {
+        int l = __ffsll((long long)pending) - 1;
+        uniq |= 1ull << l;
+        uint32_t kxy = (uint32_t)__shfl((int)xy, l), kms = (uint32_t)__shfl((int)ms, l);
+        pending &= ~__ballot(v && xy == kxy && ms == kms);
     }
-    for (uint32_t base = 0; exact && base < n_c; base += kWave) {
+    return n_coh;
+}
+
+// Leader: every candidate without pruning, find_best_match's sequential
+// rule, for NaN cost tables and the full-cost test hook.
+template <bool GUIDED>
+__device__ __forceinline__ void item_exact(const BatchArgs& a, const MapDesc* maps, uint32_t item, uint32_t rel,
+                                           uint32_t kf, uint32_t n_coh, uint32_t n_c, WaveLds* L, const float* lut,
+                                           const float* glut, int slot, int lane, float& best, uint32_t& bidx) {
+    for (uint32_t base = 0; base < n_c; base += kWave) {
         const uint32_t c = base + lane;
         const bool v = c < n_c;
         float F = 0.0f, X = -INFINITY;
@@ -762,31 +726,188 @@ __device__ void resolve_item(const BatchArgs& a, const MapDesc* maps, uint32_t i
             }
         }
     }
+}
 
-    // ---- 4. winner ----
-    if (lane == 0) {
-        uint32_t cxy, cmap, patch;
-        if (a.ext_cands) {
-            uint4 e = a.ext_cands[(size_t)rel * a.ext_cmax + bidx];
-            cxy = e.x;
-            cmap = e.y & 0x7fffffffu;
-            patch = e.z;
-        } else if (bidx < n_coh) {
-            uint4 e = L->coh[bidx];
-            cxy = e.x;
-            cmap = e.y;
-            patch = e.z;
-        } else {
-            uint2 r = a.rcands[(size_t)(item - a.rc_base) * a.m + (bidx - n_coh)];
-            cxy = r.x;
-            cmap = r.y;
-            patch = (cxy >> 16) * maps[cmap].w + (cxy & 0xffffu);
-        }
-        a.result[rel] = make_uint4(cxy, cmap | (cmap << 16), patch, __float_as_uint(best));
-        if (a.item_score) a.item_score[item] = best;
-        if (a.dbg_best) a.dbg_best[rel] = bidx;
+// Leader: the winning candidate into result[rel].
+__device__ __forceinline__ void item_winner(const BatchArgs& a, const MapDesc* maps, uint32_t item, uint32_t rel,
+                                            uint32_t n_coh, const WaveLds* L, float best, uint32_t bidx, int lane) {
+    if (lane != 0) return;
+    uint32_t cxy, cmap, patch;
+    if (a.ext_cands) {
+        uint4 e = a.ext_cands[(size_t)rel * a.ext_cmax + bidx];
+        cxy = e.x;
+        cmap = e.y & 0x7fffffffu;
+        patch = e.z;
+    } else if (bidx < n_coh) {
+        uint4 e = L->coh[bidx];
+        cxy = e.x;
+        cmap = e.y;
+        patch = e.z;
+    } else {
+        uint2 r = a.rcands[(size_t)(item - a.rc_base) * a.m + (bidx - n_coh)];
+        cxy = r.x;
+        cmap = r.y;
+        patch = (cxy >> 16) * maps[cmap].w + (cxy & 0xffffu);
     }
-    phase_clock(a, item, 6, lane);
+    a.result[rel] = make_uint4(cxy, cmap | (cmap << 16), patch, __float_as_uint(best));
+    if (a.item_score) a.item_score[item] = best;
+    if (a.dbg_best) a.dbg_best[rel] = bidx;
+}
+
+// Resolves work item `item` of the batch starting at item0 against the state
+// frozen at the batch start (n_res resolved pixels); writes result[item -
+// item0]. Lt: the team's WaveLds (member q uses Lt + q); `valid` is false
+// for members of a team past the batch end, which still take the barriers.
+template <bool GUIDED>
+__device__ void resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t item, uint32_t item0, uint32_t n_res,
+                             WaveLds* Lt, int member, int T, bool valid, const float* lut, const float* glut,
+                             int lane) {
+    WaveLds* L = Lt + member;
+    const int slot = lane % kLutCopies;
+    const uint32_t rel = item - item0;
+    const bool lead = member == 0;
+    uint32_t flat = 0;
+    int x = 0, y = 0;
+    uint32_t n_c = 0, n_k0 = 0, m_r = 0;  // candidates, host-list coherence entries, random ones
+    uint32_t r0 = 0, nrs = 0;             // this member's random slice
+    uint2 rpre = make_uint2(0, 0);
+    if (valid) {
+        if (lead) phase_clock(a, item, 0, lane);
+        flat = a.work[item];
+        x = (int)(flat % a.W);
+        y = (int)(flat / a.W);
+        if (a.ext_cands) {
+            n_c = a.ext_cand_count[rel];
+            bool k0 = false;
+            if (lane < (int)n_c) k0 = (a.ext_cands[(size_t)rel * a.ext_cmax + lane].y >> 31) == 0;
+            n_k0 = (uint32_t)__popcll(__ballot(k0));
+            m_r = n_c - n_k0;
+        } else {
+            m_r = a.m;
+        }
+        const uint32_t per = (m_r + (uint32_t)T - 1) / (uint32_t)T;
+        r0 = min(m_r, (uint32_t)member * per);
+        nrs = min(m_r, r0 + per) - r0;
+        // the member's first 64 random candidates, loaded ahead of the kNN
+        if (!a.ext_cands && !a.knn_only && (uint32_t)lane < nrs)
+            rpre = a.rcands[(size_t)(item - a.rc_base) * a.m + r0 + (uint32_t)lane];
+    }
+    // ---- 1. k nearest resolved neighbours (leader) ----
+    if (valid && lead) {
+        const uint32_t kf0 = item_knn(a, item, rel, x, y, n_res, L, lane);
+        if (lane == 0) L->t_kf = kf0;
+    }
+    if (T > 1) __syncthreads();
+    else wave_sync();
+    bool done = !valid || a.knn_only;
+    uint32_t kf = 0;
+    if (!done) {
+        kf = Lt->t_kf;
+        if (!lead && lane < (int)kf) L->nd[lane] = Lt->nd[lane];
+        wave_sync();
+        if (lead) phase_clock(a, item, 1, lane);
+        if (kf == 0) {
+            if (lead) item_at_random(a, maps, item, rel, flat, lane);
+            done = true;
+        }
+    }
+    uint32_t n_coh = 0;
+    bool exact = false;
+    if (!done) {
+        // ---- 2. neighbour state, weights, coherence candidates (every member) ----
+        uint32_t cxy, cms;
+        uint64_t uniq;
+        n_coh = item_neighbours<GUIDED>(a, maps, item, rel, x, y, kf, n_k0, L, lane, cxy, cms, uniq);
+        if (!a.ext_cands) {
+            n_k0 = n_coh;
+            n_c = n_coh + a.m;
+        }
+        if (lead) phase_clock(a, item, 2, lane);
+        exact = a.exact || a.ext_full_cost || a.lut_nan || !(L->avg > 0.0);
+        if (!exact) {
+            // ---- 3. this member's slice, exact pruning ----
+            // slice: unique coherence candidates [c0, c0 + ncs) then random ones [r0, r0 + nrs).
+            // One member: the coherence group alone first (it usually holds the
+            // winner and bounds the rest), then the random ones 64 at a time.
+            const uint32_t n_u = (uint32_t)__popcll(uniq);
+            const uint32_t cper = (n_u + (uint32_t)T - 1) / (uint32_t)T;
+            const uint32_t c0 = min(n_u, (uint32_t)member * cper);
+            const uint32_t ncs = min(n_u, c0 + cper) - c0;
+            const uint32_t nv = ncs + nrs;
+            const uint32_t upos = (uint32_t)__popcll(uniq & lanemask_lt(lane));
+            const bool mine_u = ((uniq >> lane) & 1ull) && upos >= c0 && upos < c0 + ncs;
+            Best bst{FLT_MAX, kNone};
+            uint32_t g_lo = 0;
+            bool first = true;
+            while (g_lo < nv) {
+                const uint32_t g_hi = (T == 1 && first && ncs > 0) ? ncs : min(nv, g_lo + (uint32_t)kWave);
+                // coherence entries of the group
+                if (mine_u) {
+                    const uint32_t v = upos - c0;
+                    if (v >= g_lo && v < g_hi) {
+                        L->act_xy[v - g_lo] = cxy;
+                        L->act_ms[v - g_lo] = cms;
+                        L->act_idx[v - g_lo] = (uint32_t)lane;
+                    }
+                }
+                // random entries of the group
+                const uint32_t v = g_lo + (uint32_t)lane;
+                const uint32_t rr = v >= ncs ? v - ncs : 0u;
+                const uint32_t src = rr < (uint32_t)kWave ? rr : 0u;
+                const uint32_t pxy = (uint32_t)__shfl((int)rpre.x, (int)src);
+                const uint32_t pms = (uint32_t)__shfl((int)rpre.y, (int)src);
+                if (v < g_hi && v >= ncs) {
+                    const uint32_t ci = n_k0 + r0 + rr;
+                    uint32_t rxy, rms;
+                    if (!a.ext_cands && rr < (uint32_t)kWave) {
+                        rxy = pxy;
+                        rms = pms | 0x80000000u;
+                    } else {
+                        cand_get(a, L, rel, item, n_coh, ci, rxy, rms);
+                    }
+                    L->act_xy[lane] = rxy;
+                    L->act_ms[lane] = rms;
+                    L->act_idx[lane] = ci;
+                }
+                wave_sync();
+                eval_group<GUIDED>(maps, L, lut, glut, slot, lane, kf, g_hi - g_lo, bst);
+                if (lead && first) phase_clock(a, item, 4, lane);
+                first = false;
+                g_lo = g_hi;
+            }
+            if (lead) phase_clock(a, item, 5, lane);
+            if (lane == 0) {
+                L->t_B = bst.B;
+                L->t_Bi = bst.Bi;
+            }
+        } else if (lead) {
+            float best = FLT_MAX;
+            uint32_t bidx = 0;
+            item_exact<GUIDED>(a, maps, item, rel, kf, n_coh, n_c, L, lut, glut, slot, lane, best, bidx);
+            item_winner(a, maps, item, rel, n_coh, L, best, bidx, lane);
+        }
+    }
+    if (T > 1) __syncthreads();
+    else wave_sync();
+    // ---- 4. winner: lexicographic (cost, index) minimum over the members ----
+    if (!done && !exact && lead) {
+        float best = FLT_MAX;
+        uint32_t bidx = kNone;
+        for (int q = 0; q < T; ++q) {
+            const float B = Lt[q].t_B;
+            const uint32_t Bi = Lt[q].t_Bi;
+            if (Bi != kNone && (bidx == kNone || B < best || (B == best && Bi < bidx))) {
+                best = B;
+                bidx = Bi;
+            }
+        }
+        if (bidx == kNone) {
+            best = FLT_MAX;
+            bidx = 0;
+        }
+        item_winner(a, maps, item, rel, n_coh, L, best, bidx, lane);
+    }
+    if (valid && lead) phase_clock(a, item, 6, lane);
 }
 
 
@@ -842,8 +963,10 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve(BatchArgs a) {
     WaveLds* wls = setup_lds<GUIDED>(a, smem, lut, glut, smaps, maps);
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
-    const uint32_t item = a.item0 + blockIdx.x * kWavesPerWG + wave;
-    if (item < a.item1) resolve_item<GUIDED>(a, maps, item, a.item0, a.n_res, wls + wave, lut, glut, lane);
+    const int T = a.team > 1 ? (int)a.team : 1;
+    const int team = wave / T, member = wave % T;
+    const uint32_t item = a.item0 + blockIdx.x * (uint32_t)(kWavesPerWG / T) + (uint32_t)team;
+    resolve_team<GUIDED>(a, maps, item, a.item0, a.n_res, wls + team * T, member, T, item < a.item1, lut, glut, lane);
     if (!a.fused_commit) return;
     __threadfence();
     __syncthreads();
@@ -871,14 +994,60 @@ __global__ __launch_bounds__(kWave* kSmallWaves) void k_resolve_small(BatchArgs 
     const int lane = threadIdx.x % kWave;
     for (uint32_t b = 0; b < n_batches; ++b) {
         const uint32_t i0 = bounds[b], i1 = bounds[b + 1];
-        const uint32_t item = i0 + (uint32_t)wave;
-        if (item < i1) resolve_item<GUIDED>(a, maps, item, i0, nres[b], wls + wave, lut, glut, lane);
+        // team size: the largest power of two with one team per item
+        int T = kSmallWaves;
+        while (T > 1 && (uint32_t)T * (i1 - i0) > (uint32_t)kSmallWaves) T >>= 1;
+        const int team = wave / T, member = wave % T;
+        const uint32_t item = i0 + (uint32_t)team;
+        resolve_team<GUIDED>(a, maps, item, i0, nres[b], wls + team * T, member, T, item < i1, lut, glut, lane);
         __syncthreads();
         const uint32_t ci = i0 + threadIdx.x;
         if (ci < i1) commit_one(a.W, a.tiles_x, maps, a.work, a.result, ci, i0, a.first_new, a.state_w, a.mask_w,
                                 a.score_w);
         __syncthreads();
     }
+}
+
+// Index u with start[u] <= g < start[u + 1] (start[0] = 0, n entries).
+__device__ __forceinline__ uint32_t find_unit(const uint32_t* start, uint32_t n, uint32_t g) {
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// One round of the multi-job engine: unit u is one job's current batch;
+// its workgroups are [wg_start[u], wg_start[u + 1]).
+template <bool GUIDED>
+__global__ __launch_bounds__(kWave* kWavesPerWG, 4) void k_resolve_multi(const BatchArgs* units, const uint32_t* wg_start,
+                                                                      uint32_t n_units) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ MapDesc smaps[kMapCache];
+    const uint32_t u = find_unit(wg_start, n_units, blockIdx.x);
+    const BatchArgs& a = units[u];
+    float *lut, *glut;
+    const MapDesc* maps;
+    WaveLds* wls = setup_lds<GUIDED>(a, smem, lut, glut, smaps, maps);
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int T = a.team > 1 ? (int)a.team : 1;
+    const int team = wave / T, member = wave % T;
+    const uint32_t item = a.item0 + (blockIdx.x - wg_start[u]) * (uint32_t)(kWavesPerWG / T) + (uint32_t)team;
+    resolve_team<GUIDED>(a, maps, item, a.item0, a.n_res, wls + team * T, member, T, item < a.item1, lut, glut, lane);
+}
+
+// Commit of every unit of a round: thread g commits item g - item_start[u].
+__global__ void k_commit_multi(const BatchArgs* units, const uint32_t* item_start, uint32_t n_units,
+                               uint32_t total) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const uint32_t u = find_unit(item_start, n_units, g);
+    const BatchArgs& a = units[u];
+    commit_one(a.W, a.tiles_x, a.maps, a.work, a.result, a.item0 + (g - item_start[u]), a.item0, a.first_new,
+               a.state_w, a.mask_w, a.score_w);
 }
 
 // Random candidates of items [item0, item1): Pcg32::seed_from_u64(stage
@@ -1004,6 +1173,12 @@ hipError_t configure_kernels() {
     e = hipFuncSetAttribute((const void*)k_resolve_small<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)resolve_small_lds_bytes(true));
     if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)k_resolve_multi<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)resolve_lds_bytes(true));
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)k_resolve_multi<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)resolve_lds_bytes(false));
+    if (e != hipSuccess) return e;
     return hipFuncSetAttribute((const void*)k_resolve_small<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)resolve_small_lds_bytes(false));
 }
@@ -1025,6 +1200,37 @@ size_t resolve_small_lds_bytes(bool guided) {
 }
 
 uint32_t small_batch_max() { return kSmallWaves; }
+uint32_t items_per_wg(uint32_t team) { return kWavesPerWG / (team > 1 ? team : 1); }
+
+// Waves per item for a batch of n items: while the batch leaves most of the
+// device's wave slots idle (about 4096 resident waves of the resolve
+// kernel), each item gets a team of up to a workgroup.
+uint32_t team_for(uint64_t n) {
+    uint32_t T = kWavesPerWG;
+    while (T > 1 && (uint64_t)T * n > 4096ull) T >>= 1;
+    return T;
+}
+
+hipError_t launch_resolve_multi(const BatchArgs* units, const uint32_t* wg_start, uint32_t n_units, uint32_t n_wg,
+                                bool guided, hipStream_t s) {
+    if (n_units == 0 || n_wg == 0) return hipSuccess;
+    size_t lds = resolve_lds_bytes(guided);
+    if (guided) {
+        hipLaunchKernelGGL(k_resolve_multi<true>, dim3(n_wg), dim3(kWave * kWavesPerWG), lds, s, units, wg_start,
+                           n_units);
+    } else {
+        hipLaunchKernelGGL(k_resolve_multi<false>, dim3(n_wg), dim3(kWave * kWavesPerWG), lds, s, units, wg_start,
+                           n_units);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_commit_multi(const BatchArgs* units, const uint32_t* item_start, uint32_t n_units, uint32_t total,
+                               hipStream_t s) {
+    if (n_units == 0 || total == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_commit_multi, dim3((total + 255) / 256), dim3(256), 0, s, units, item_start, n_units, total);
+    return hipGetLastError();
+}
 
 hipError_t launch_resolve_small(const BatchArgs& a, const uint32_t* bounds, const uint32_t* nres,
                                 uint32_t n_batches, bool guided, hipStream_t s) {
@@ -1043,7 +1249,9 @@ hipError_t launch_resolve_small(const BatchArgs& a, const uint32_t* bounds, cons
 hipError_t launch_resolve(const BatchArgs& a, bool guided, hipStream_t s) {
     uint32_t n = a.item1 - a.item0;
     if (n == 0) return hipSuccess;
-    dim3 grid((n + kWavesPerWG - 1) / kWavesPerWG), block(kWave * kWavesPerWG);
+    if (a.team > kWavesPerWG || (a.team & (a.team - 1)) != 0) return hipErrorInvalidValue;
+    const uint32_t ipw = items_per_wg(a.team);
+    dim3 grid((n + ipw - 1) / ipw), block(kWave * kWavesPerWG);
     size_t lds = resolve_lds_bytes(guided);
     if (guided) {
         hipLaunchKernelGGL(k_resolve<true>, grid, block, lds, s, a);

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -288,6 +289,20 @@ struct tsg_job {
     std::vector<std::vector<float>> lut_keep;
     bool ran = false;
     bool state_ready = false;
+    size_t rc_budget = 32u << 20;  // bytes of random candidates generated per chunk
+    std::vector<uint8_t> picks_ready;
+    uint64_t overall_total = 0, overall_cur = 0;
+    hipEvent_t ev_run0 = nullptr, ev_run1 = nullptr;
+    // cursor of the resumable stage driver
+    struct Step {
+        bool active = false, open = false;
+        int t = 0;
+        uint32_t level = 0;
+        uint64_t pss = 0, S = 0, S_eff = 0, redo = 0, redo_eff = 0, i = 0;
+        uint64_t rc_lo = 0, rc_hi = 0;
+        uint64_t planned_res = 0;
+        BatchArgs a{};
+    } step;
     double run_ms = 0.0;
     uint64_t n_batches = 0;
     // optional per-kernel device timing (events on the job's stream)
@@ -307,6 +322,8 @@ struct tsg_job {
     uint64_t kern_launches[5] = {0, 0, 0, 0, 0};
     ~tsg_job() {
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+        if (ev_run0) (void)hipEventDestroy(ev_run0);
+        if (ev_run1) (void)hipEventDestroy(ev_run1);
     }
 };
 
@@ -445,6 +462,7 @@ uint64_t batch_end(const tsg_job* j, uint64_t i, uint64_t redo, uint64_t S_eff) 
 
 enum { K_RESOLVE = 0, K_COMMIT = 1, K_RAND = 2, K_RECOLOR = 3, K_STATS = 4 };
 constexpr uint64_t kFuseMax = 2048;  // batches up to this size commit inside the resolve launch
+constexpr size_t kEngineRcBudget = 4u << 20;  // random-candidate chunk of a job under the engine
 
 hipError_t next_event(tsg_job* j, hipEvent_t* e) {
     if (j->ev_next == j->ev_pool.size()) {
@@ -472,203 +490,208 @@ hipError_t timed_launch(tsg_job* j, int kind, uint64_t items, F&& launch) {
     return hipSuccess;
 }
 
-int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
+// ---- resumable stage driver (Generator::resolve, ms.rs:786-1050) ----
+
+// Pick order of stage t's new pixels (computed once; the chain depends only
+// on counts and seeds, not on image content).
+void stage_picks(tsg_job* j, int t) {
+    tsg_job::Step& st = j->step;
+    if (j->picks_ready[t]) return;
+    const int P = (int)j->prm.backtrack_stages;
+    const int ps = P - t;
+    const uint64_t pss = (uint64_t)Pcg32::seed_from_u64(j->prm.seed + (uint64_t)ps).next_u32();
+    const uint64_t S = stage_size(j, ps);
+    const uint64_t redo = st.planned_res - j->locked;
+    if (S > redo) j->picks_keep[t] = pick_new(j, pss, redo, S - redo);
+    st.planned_res += j->picks_keep[t].size();
+    j->picks_ready[t] = 1;
+}
+
+int step_begin(tsg_job* j) {
     hipStream_t s = j->ctx->stream;
     if (!j->state_ready) return fail(TSG_ERR_INVALID_ARGUMENT, "job state was not initialised");
-    int rc = TSG_OK;
+    if (j->ran || j->step.active) return fail(TSG_ERR_INVALID_ARGUMENT, "a job runs once (Session::run consumes the session)");
     j->timed.clear();
     j->ev_next = 0;
     for (int q = 0; q < 5; ++q) j->kern_ms[q] = 0.0, j->kern_items[q] = 0, j->kern_launches[q] = 0;
     const int P = (int)j->prm.backtrack_stages;
-    const uint32_t nf = (uint32_t)j->filt.size();
-    const uint32_t m = (uint32_t)j->prm.random_sample_locations;
     j->st_items.assign(P + 1, 0);
     HIP_TRY(hipMemsetAsync(j->st_sum.p, 0, (P + 1) * sizeof(double), s));
     HIP_TRY(hipMemsetAsync(j->st_cnt.p, 0, (P + 1) * sizeof(unsigned long long), s));
     HIP_TRY(hipMemsetAsync(j->overflow.p, 0, 4, s));
-    j->picks_keep.assign(P + 1, {});
+    HIP_TRY(hipMemsetAsync(j->done_counter.p, 0, 4, s));
+    if (j->picks_keep.size() != (size_t)P + 1) {
+        j->picks_keep.assign(P + 1, {});
+        j->picks_ready.assign(P + 1, 0);
+    }
     j->lut_keep.assign(P + 1, std::vector<float>(512));
     j->small_keep.clear();
     j->small_used = 0;
     j->n_batches = 0;
-    HIP_TRY(hipMemsetAsync(j->done_counter.p, 0, 4, s));
     // Table arena of the small-batch runs: a run of nb batches takes 2 nb + 1
     // entries. Batches of at most 16 items occur only while
     // (resolved - locked) >> shift <= 16, i.e. for at most 17 << shift
     // batches per segment, or for every batch when px_per_batch <= 16.
-    {
-        uint64_t items = 0;
-        for (int t = 0; t <= P; ++t) items += stage_size(j, P - t);
-        uint64_t bound = items;
-        if (j->sched.px_per_batch == 0 || j->sched.px_per_batch > small_batch_max()) {
-            uint32_t sh = std::max(j->sched.redo_shift, j->sched.new_shift);
-            uint64_t per_seg = sh >= 40 ? items : (uint64_t)(small_batch_max() + 1) << sh;
-            bound = std::min<uint64_t>(items, 2ull * (P + 1) * per_seg);
-        }
-        HIP_TRY(j->small_tab.reserve(3 * bound + 4 * (P + 1) + 16));
+    uint64_t items = 0;
+    for (int t = 0; t <= P; ++t) items += stage_size(j, P - t);
+    uint64_t bound = items;
+    if (j->sched.px_per_batch == 0 || j->sched.px_per_batch > small_batch_max()) {
+        uint32_t sh = std::max(j->sched.redo_shift, j->sched.new_shift);
+        uint64_t per_seg = sh >= 40 ? items : (uint64_t)(small_batch_max() + 1) << sh;
+        bound = std::min<uint64_t>(items, 2ull * (P + 1) * per_seg);
     }
+    HIP_TRY(j->small_tab.reserve(3 * bound + 4 * (P + 1) + 16));
+    // largest batch: a fraction of the resolved count, a fixed size, or the cap
+    uint64_t maxb = j->sched.px_per_batch ? j->sched.px_per_batch
+                                          : (uint64_t)j->N >> std::min(j->sched.redo_shift, j->sched.new_shift);
+    if (j->sched.max_batch) maxb = std::min<uint64_t>(maxb, j->sched.max_batch);
+    maxb = std::max<uint64_t>(maxb + 1, small_batch_max());
+    HIP_TRY(j->result.reserve(std::min<uint64_t>(maxb, j->N)));
+    j->overall_total = items;
+    j->overall_cur = 0;
+    j->step = tsg_job::Step{};
+    j->step.active = true;
+    j->step.planned_res = j->n_res;
+    HIP_TRY(hipEventCreate(&j->ev_run0));
+    HIP_TRY(hipEventCreate(&j->ev_run1));
+    HIP_TRY(hipEventRecord(j->ev_run0, s));
+    return TSG_OK;
+}
 
-    uint64_t overall_total = 0;
-    for (int t = 0; t <= P; ++t) overall_total += stage_size(j, P - t);
-    uint64_t overall_cur = 0;
-    int last_pct = 0;
-    std::vector<uint8_t> snap;
-
-    hipEvent_t ev0, ev1;
-    HIP_TRY(hipEventCreate(&ev0));
-    HIP_TRY(hipEventCreate(&ev1));
-    HIP_TRY(hipEventRecord(ev0, s));
-
-    for (int t = 0; t <= P; ++t) {
-        const int ps = P - t;
-        const uint32_t level = t == 0 ? 0u : (uint32_t)std::min(t, P - 1);
-        const MapDesc* maps = j->maps.p + (size_t)level * nf;
-        if (level > 0) {
-            RecolorArgs ra{maps, j->rlist.p, (uint32_t)j->n_res, j->state.p};
-            HIP_TRY(timed_launch(j, K_RECOLOR, j->n_res, [&] { return launch_recolor(ra, s); }));
-        }
-        const uint64_t pss = (uint64_t)Pcg32::seed_from_u64(j->prm.seed + (uint64_t)ps).next_u32();
-        const uint64_t S = stage_size(j, ps);
-        const uint64_t redo = j->n_res - j->locked;
-        float alpha_s = 0.0f;
-        if (j->guided && ps > 0) {
-            float b = j->prm.guide_alpha * (1.0f - ((float)j->n_res / (float)j->N0));
-            alpha_s = (b * b) * b;
-        }
-        stage_luts(j->prm.cauchy_dispersion, j->guided, alpha_s, j->lut_keep[t].data());
-        uint32_t lut_nan = 0;
-        for (float v : j->lut_keep[t]) lut_nan |= (uint32_t)std::isnan(v);
-        HIP_TRY(hipMemcpyAsync(j->lut.p, j->lut_keep[t].data(), 512 * 4, hipMemcpyHostToDevice, s));
-        uint64_t S_eff = S;
-        if (S > redo) {
-            j->picks_keep[t] = pick_new(j, pss, redo, S - redo);
-            S_eff = redo + j->picks_keep[t].size();
-            if (!j->picks_keep[t].empty())
-                HIP_TRY(hipMemcpyAsync(j->rlist.p + j->locked + redo, j->picks_keep[t].data(),
-                                       j->picks_keep[t].size() * 4, hipMemcpyHostToDevice, s));
-        }
-        const uint64_t redo_eff = std::min(redo, S_eff);
-        const uint32_t* work = j->rlist.p + j->locked;
-        uint64_t rc_lo = 0, rc_hi = 0;
-        // random candidates must cover items [lo, hi)
-        auto ensure_rc = [&](uint64_t lo, uint64_t hi) -> int {
-            if (lo >= rc_lo && hi <= rc_hi) return TSG_OK;
-            uint64_t cap_items = std::max<uint64_t>(hi - lo, (uint64_t)(32u << 20) / (8ull * std::max(1u, m)));
-            rc_lo = lo;
-            rc_hi = std::min<uint64_t>(S_eff, lo + cap_items);
-            HIP_TRY(j->rc.reserve((rc_hi - rc_lo) * m));
-            RandArgs ra{maps, nf, pss, (uint32_t)rc_lo, (uint32_t)rc_hi, m, j->rc.p, j->overflow.p};
-            HIP_TRY(timed_launch(j, K_RAND, rc_hi - rc_lo, [&] { return launch_rand(ra, s); }));
-            return TSG_OK;
-        };
-        BatchArgs a{};
-        a.W = j->W;
-        a.H = j->H;
-        a.tiles_x = j->tiles_x;
-        a.tiles_y = j->tiles_y;
-        a.mask = j->mask.p;
-        a.state = j->state.p;
-        a.tg = j->guided ? j->tg.p + (size_t)level * j->N : nullptr;
-        a.maps = maps;
-        a.n_maps = nf;
-        a.work = work;
-        a.rlist = j->rlist.p;
-        a.m = m;
-        a.offtab = j->ctx->offtab.p;
-        a.offtab_n = j->ctx->offtab_n;
-        a.lut = j->lut.p;
-        a.stage_seed = pss;
-        a.k = j->prm.nearest_neighbors;
-        a.item_score = j->item_score.p;
-        a.lut_nan = lut_nan;
-        a.first_new = (uint32_t)redo_eff;
-        a.done_counter = j->done_counter.p;
-        a.dbg_clk = (j->profile_level >= 2 && t == 0) ? j->clk.p : nullptr;
-        a.state_w = j->state.p;
-        a.mask_w = j->mask.p;
-        a.score_w = j->score.p;
-        // consecutive small batches go to one single-workgroup launch
-        std::vector<uint32_t> sb_bounds, sb_nres;
-        auto flush_small = [&]() -> int {
-            if (sb_nres.empty()) return TSG_OK;
-            int rc2 = ensure_rc(sb_bounds.front(), sb_bounds.back());
-            if (rc2) return rc2;
-            j->small_keep.emplace_back(sb_bounds);
-            std::vector<uint32_t>& tab = j->small_keep.back();
-            tab.insert(tab.end(), sb_nres.begin(), sb_nres.end());
-            const size_t nb = sb_nres.size();
-            HIP_TRY(j->result.reserve(small_batch_max()));
-            // the table lives in a per-run device arena: bounds (nb+1) then nres (nb)
-            uint32_t* dt = j->small_tab.p + j->small_used;
-            if (j->small_used + tab.size() > j->small_tab.n) return fail(TSG_ERR_DEVICE, "small-batch table overflow");
-            j->small_used += tab.size();
-            HIP_TRY(hipMemcpyAsync(dt, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, s));
-            BatchArgs sa2 = a;
-            sa2.rcands = j->rc.p;
-            sa2.rc_base = (uint32_t)rc_lo;
-            sa2.result = j->result.p;
-            HIP_TRY(timed_launch(j, K_RESOLVE, sb_bounds.back() - sb_bounds.front(), [&] {
-                return launch_resolve_small(sa2, dt, dt + nb + 1, (uint32_t)nb, j->guided, s);
-            }));
-            sb_bounds.clear();
-            sb_nres.clear();
-            return TSG_OK;
-        };
-        for (uint64_t i = 0; i < S_eff;) {
-            uint64_t e = batch_end(j, i, redo, S_eff);
-            const uint64_t n_new = e > redo_eff ? e - std::max(i, redo_eff) : 0;
-            if (e - i <= small_batch_max()) {
-                if (sb_bounds.empty()) sb_bounds.push_back((uint32_t)i);
-                sb_bounds.push_back((uint32_t)e);
-                sb_nres.push_back((uint32_t)j->n_res);
-            } else {
-                if ((rc = flush_small())) return rc;
-                if ((rc = ensure_rc(i, e))) return rc;
-                HIP_TRY(j->result.reserve(e - i));
-                a.n_res = (uint32_t)j->n_res;
-                a.rcands = j->rc.p;
-                a.rc_base = (uint32_t)rc_lo;
-                a.item0 = (uint32_t)i;
-                a.item1 = (uint32_t)e;
-                a.result = j->result.p;
-                a.fused_commit = (e - i) <= kFuseMax;
-                HIP_TRY(timed_launch(j, K_RESOLVE, e - i, [&] { return launch_resolve(a, j->guided, s); }));
-                if (!a.fused_commit) {
-                    CommitArgs ca{j->W, j->tiles_x, maps, work, j->result.p, (uint32_t)i, (uint32_t)e,
-                                  (uint32_t)redo_eff, j->state.p, j->mask.p, j->score.p};
-                    HIP_TRY(timed_launch(j, K_COMMIT, e - i, [&] { return launch_commit(ca, s); }));
-                }
-            }
-            j->n_res += n_new;
-            j->n_batches += 1;
-            i = e;
-            if (cb) {
-                if ((rc = flush_small())) return rc;
-                int pct = (int)std::lround((double)(overall_cur + i) / (double)std::max<uint64_t>(overall_total, 1) *
-                                           100.0);
-                if (pct != last_pct) {
-                    last_pct = pct;
-                    snap.resize(j->N * 4);
-                    HIP_TRY(hipStreamSynchronize(s));
-                    std::vector<uint4> stv(j->N);
-                    HIP_TRY(hipMemcpy(stv.data(), j->state.p, j->N * sizeof(uint4), hipMemcpyDeviceToHost));
-                    for (size_t q = 0; q < j->N; ++q) memcpy(&snap[q * 4], &stv[q].x, 4);
-                    cb(user, snap.data(), j->W, j->H, overall_cur + i, overall_total, i, S);
-                }
-            }
-        }
-        if ((rc = flush_small())) return rc;
-        StatsArgs sa{j->item_score.p, (uint32_t)S_eff, j->st_sum.p + t, j->st_cnt.p + t};
-        HIP_TRY(timed_launch(j, K_STATS, S_eff, [&] { return launch_stats(sa, s); }));
-        j->st_items[t] = S_eff;
-        overall_cur += S;
+// Stage preamble (ms.rs:786-858): level, recolour, seed, size, alpha, LUTs, picks.
+int step_open_stage(tsg_job* j) {
+    hipStream_t s = j->ctx->stream;
+    tsg_job::Step& st = j->step;
+    const int P = (int)j->prm.backtrack_stages;
+    const int t = st.t;
+    const int ps = P - t;
+    const uint32_t nf = (uint32_t)j->filt.size();
+    const uint32_t m = (uint32_t)j->prm.random_sample_locations;
+    st.level = t == 0 ? 0u : (uint32_t)std::min(t, P - 1);
+    const MapDesc* maps = j->maps.p + (size_t)st.level * nf;
+    if (st.level > 0) {
+        RecolorArgs ra{maps, j->rlist.p, (uint32_t)j->n_res, j->state.p};
+        HIP_TRY(timed_launch(j, K_RECOLOR, j->n_res, [&] { return launch_recolor(ra, s); }));
     }
-    HIP_TRY(hipEventRecord(ev1, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    st.pss = (uint64_t)Pcg32::seed_from_u64(j->prm.seed + (uint64_t)ps).next_u32();
+    st.S = stage_size(j, ps);
+    st.redo = j->n_res - j->locked;
+    float alpha_s = 0.0f;
+    if (j->guided && ps > 0) {
+        float b = j->prm.guide_alpha * (1.0f - ((float)j->n_res / (float)j->N0));
+        alpha_s = (b * b) * b;
+    }
+    stage_luts(j->prm.cauchy_dispersion, j->guided, alpha_s, j->lut_keep[t].data());
+    uint32_t lut_nan = 0;
+    for (float v : j->lut_keep[t]) lut_nan |= (uint32_t)std::isnan(v);
+    HIP_TRY(hipMemcpyAsync(j->lut.p, j->lut_keep[t].data(), 512 * 4, hipMemcpyHostToDevice, s));
+    if (!j->picks_ready[t]) {
+        // picks are computed in stage order; planned_res tracks the count
+        st.planned_res = j->n_res;
+        stage_picks(j, t);
+    }
+    st.S_eff = st.S;
+    if (st.S > st.redo) {
+        st.S_eff = st.redo + j->picks_keep[t].size();
+        if (!j->picks_keep[t].empty())
+            HIP_TRY(hipMemcpyAsync(j->rlist.p + j->locked + st.redo, j->picks_keep[t].data(),
+                                   j->picks_keep[t].size() * 4, hipMemcpyHostToDevice, s));
+    }
+    st.redo_eff = std::min(st.redo, st.S_eff);
+    st.i = 0;
+    st.rc_lo = st.rc_hi = 0;
+    BatchArgs& a = st.a;
+    a = BatchArgs{};
+    a.W = j->W;
+    a.H = j->H;
+    a.tiles_x = j->tiles_x;
+    a.tiles_y = j->tiles_y;
+    a.mask = j->mask.p;
+    a.state = j->state.p;
+    a.tg = j->guided ? j->tg.p + (size_t)st.level * j->N : nullptr;
+    a.maps = maps;
+    a.n_maps = nf;
+    a.work = j->rlist.p + j->locked;
+    a.rlist = j->rlist.p;
+    a.m = m;
+    a.offtab = j->ctx->offtab.p;
+    a.offtab_n = j->ctx->offtab_n;
+    a.lut = j->lut.p;
+    a.stage_seed = st.pss;
+    a.k = j->prm.nearest_neighbors;
+    a.item_score = j->item_score.p;
+    a.lut_nan = lut_nan;
+    a.first_new = (uint32_t)st.redo_eff;
+    a.done_counter = j->done_counter.p;
+    a.dbg_clk = (j->profile_level >= 2 && t == 0) ? j->clk.p : nullptr;
+    a.state_w = j->state.p;
+    a.mask_w = j->mask.p;
+    a.score_w = j->score.p;
+    a.result = j->result.p;
+    st.open = true;
+    return TSG_OK;
+}
+
+// Random candidates must cover items [lo, hi) of the open stage.
+int step_ensure_rc(tsg_job* j, uint64_t lo, uint64_t hi) {
+    tsg_job::Step& st = j->step;
+    if (lo >= st.rc_lo && hi <= st.rc_hi) return TSG_OK;
+    const uint32_t m = (uint32_t)j->prm.random_sample_locations;
+    uint64_t cap_items = std::max<uint64_t>(hi - lo, (uint64_t)j->rc_budget / (8ull * std::max(1u, m)));
+    st.rc_lo = lo;
+    st.rc_hi = std::min<uint64_t>(st.S_eff, lo + cap_items);
+    HIP_TRY(j->rc.reserve((st.rc_hi - st.rc_lo) * m));
+    RandArgs ra{st.a.maps, st.a.n_maps, st.pss, (uint32_t)st.rc_lo, (uint32_t)st.rc_hi, m, j->rc.p, j->overflow.p};
+    HIP_TRY(timed_launch(j, K_RAND, st.rc_hi - st.rc_lo, [&] { return launch_rand(ra, j->ctx->stream); }));
+    st.a.rcands = j->rc.p;
+    st.a.rc_base = (uint32_t)st.rc_lo;
+    return TSG_OK;
+}
+
+// Next batch [i0, i1) of the open stage; false at the stage end.
+bool step_peek(tsg_job* j, uint64_t& i0, uint64_t& i1) {
+    tsg_job::Step& st = j->step;
+    if (st.i >= st.S_eff) return false;
+    i0 = st.i;
+    i1 = batch_end(j, st.i, st.redo, st.S_eff);
+    return true;
+}
+
+// The batch [i0, i1) has been enqueued: advance the cursor and counts.
+void step_advance(tsg_job* j, uint64_t i0, uint64_t i1) {
+    tsg_job::Step& st = j->step;
+    const uint64_t n_new = i1 > st.redo_eff ? i1 - std::max(i0, st.redo_eff) : 0;
+    j->n_res += n_new;
+    j->n_batches += 1;
+    st.i = i1;
+}
+
+int step_close_stage(tsg_job* j) {
+    tsg_job::Step& st = j->step;
+    StatsArgs sa{j->item_score.p, (uint32_t)st.S_eff, j->st_sum.p + st.t, j->st_cnt.p + st.t};
+    HIP_TRY(timed_launch(j, K_STATS, st.S_eff, [&] { return launch_stats(sa, j->ctx->stream); }));
+    j->st_items[st.t] = st.S_eff;
+    j->overall_cur += st.S;
+    st.open = false;
+    st.t += 1;
+    return TSG_OK;
+}
+
+bool step_done(const tsg_job* j) { return j->step.t > (int)j->prm.backtrack_stages; }
+
+// After the job's last launch: record the end event; step_finish waits.
+int step_end(tsg_job* j) {
+    HIP_TRY(hipEventRecord(j->ev_run1, j->ctx->stream));
+    return TSG_OK;
+}
+
+int step_finish(tsg_job* j) {
+    HIP_TRY(hipEventSynchronize(j->ev_run1));
     float ms = 0.0f;
-    HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
+    HIP_TRY(hipEventElapsedTime(&ms, j->ev_run0, j->ev_run1));
+    (void)hipEventDestroy(j->ev_run0);
+    (void)hipEventDestroy(j->ev_run1);
+    j->ev_run0 = j->ev_run1 = nullptr;
     j->run_ms = ms;
     for (const tsg_job::Timed& t : j->timed) {
         float km = 0.0f;
@@ -679,8 +702,288 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
     }
     uint32_t ovf = 0;
     HIP_TRY(hipMemcpy(&ovf, j->overflow.p, 4, hipMemcpyDeviceToHost));
-    if (ovf) return fail(TSG_ERR_SAMPLING, "a random-candidate rejection loop hit its bound (sample mask too sparse)");
+    j->step.active = false;
     j->ran = true;
+    if (ovf) return fail(TSG_ERR_SAMPLING, "a random-candidate rejection loop hit its bound (sample mask too sparse)");
+    return TSG_OK;
+}
+
+// Single job: runs of small batches go to one single-workgroup launch; other
+// batches get one k_resolve launch, committing in its last workgroup when
+// the batch holds at most kFuseMax items.
+int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
+    hipStream_t s = j->ctx->stream;
+    int rc;
+    if ((rc = step_begin(j))) return rc;
+    int last_pct = 0;
+    std::vector<uint8_t> snap;
+    std::vector<uint32_t> sb_bounds, sb_nres;
+    auto flush_small = [&]() -> int {
+        if (sb_nres.empty()) return TSG_OK;
+        int rc2 = step_ensure_rc(j, sb_bounds.front(), sb_bounds.back());
+        if (rc2) return rc2;
+        j->small_keep.emplace_back(sb_bounds);
+        std::vector<uint32_t>& tab = j->small_keep.back();
+        tab.insert(tab.end(), sb_nres.begin(), sb_nres.end());
+        const size_t nb = sb_nres.size();
+        // the table lives in a per-run device arena: bounds (nb+1) then nres (nb)
+        if (j->small_used + tab.size() > j->small_tab.n) return fail(TSG_ERR_DEVICE, "small-batch table overflow");
+        uint32_t* dt = j->small_tab.p + j->small_used;
+        j->small_used += tab.size();
+        HIP_TRY(hipMemcpyAsync(dt, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, s));
+        const BatchArgs& a = j->step.a;
+        HIP_TRY(timed_launch(j, K_RESOLVE, sb_bounds.back() - sb_bounds.front(), [&] {
+            return launch_resolve_small(a, dt, dt + nb + 1, (uint32_t)nb, j->guided, s);
+        }));
+        sb_bounds.clear();
+        sb_nres.clear();
+        return TSG_OK;
+    };
+    while (!step_done(j)) {
+        if ((rc = step_open_stage(j))) return rc;
+        uint64_t i0, i1;
+        while (step_peek(j, i0, i1)) {
+            if (i1 - i0 <= small_batch_max()) {
+                if (sb_bounds.empty()) sb_bounds.push_back((uint32_t)i0);
+                sb_bounds.push_back((uint32_t)i1);
+                sb_nres.push_back((uint32_t)j->n_res);
+            } else {
+                if ((rc = flush_small())) return rc;
+                if ((rc = step_ensure_rc(j, i0, i1))) return rc;
+                BatchArgs& a = j->step.a;
+                a.n_res = (uint32_t)j->n_res;
+                a.item0 = (uint32_t)i0;
+                a.item1 = (uint32_t)i1;
+                a.fused_commit = (i1 - i0) <= kFuseMax;
+                a.team = team_for(i1 - i0);
+                HIP_TRY(timed_launch(j, K_RESOLVE, i1 - i0, [&] { return launch_resolve(a, j->guided, s); }));
+                if (!a.fused_commit) {
+                    CommitArgs ca{j->W, j->tiles_x, a.maps, a.work, j->result.p, (uint32_t)i0, (uint32_t)i1,
+                                  a.first_new, j->state.p, j->mask.p, j->score.p};
+                    HIP_TRY(timed_launch(j, K_COMMIT, i1 - i0, [&] { return launch_commit(ca, s); }));
+                }
+            }
+            step_advance(j, i0, i1);
+            if (cb) {
+                if ((rc = flush_small())) return rc;
+                int pct = (int)std::lround((double)(j->overall_cur + i1) /
+                                           (double)std::max<uint64_t>(j->overall_total, 1) * 100.0);
+                if (pct != last_pct) {
+                    last_pct = pct;
+                    snap.resize(j->N * 4);
+                    HIP_TRY(hipStreamSynchronize(s));
+                    std::vector<uint4> stv(j->N);
+                    HIP_TRY(hipMemcpy(stv.data(), j->state.p, j->N * sizeof(uint4), hipMemcpyDeviceToHost));
+                    for (size_t q = 0; q < j->N; ++q) memcpy(&snap[q * 4], &stv[q].x, 4);
+                    cb(user, snap.data(), j->W, j->H, j->overall_cur + i1, j->overall_total, i1, j->step.S);
+                }
+            }
+        }
+        if ((rc = flush_small())) return rc;
+        if ((rc = step_close_stage(j))) return rc;
+    }
+    if ((rc = step_end(j))) return rc;
+    return step_finish(j);
+}
+
+// ---- multi-job engine ----
+
+// All stages' pick orders of a job (content independent; computed ahead on
+// host threads when the job is admitted to the engine).
+void precompute_picks(tsg_job* j) {
+    const int P = (int)j->prm.backtrack_stages;
+    if (j->picks_keep.size() != (size_t)P + 1) {
+        j->picks_keep.assign(P + 1, {});
+        j->picks_ready.assign(P + 1, 0);
+    }
+    j->step.planned_res = j->n_res;
+    for (int t = 0; t <= P; ++t) stage_picks(j, t);
+}
+
+size_t job_footprint(const tsg_job* j) {
+    size_t b = 0;
+    for (auto& e : j->ex) b += e->n * 4;
+    for (auto& e : j->exg) b += e->n * 4;
+    for (auto& e : j->smask) b += e->n;
+    b += j->tg.n * 4 + j->maps.n * sizeof(MapDesc);
+    b += j->N * (sizeof(uint4) + 4 + 4 + 4) + (size_t)j->tiles_x * j->tiles_y * 8;
+    b += j->result.n * sizeof(uint4) + j->rc.n * sizeof(uint2) + j->small_tab.n * 4;
+    return b;
+}
+
+}  // namespace
+
+struct tsg_batch {
+    tsg_ctx* ctx = nullptr;
+    uint32_t max_active = 0;
+    struct Entry {
+        uint64_t ticket;
+        tsg_job* job;
+        int status;
+    };
+    std::vector<Entry> active, pending, done, taken;
+    uint64_t next_ticket = 1;
+    static constexpr int kRing = 8;
+    // per-round tables: units of both kinds, then the workgroup and item starts
+    BatchArgs* h_units[kRing] = {};
+    uint32_t* h_idx[kRing] = {};
+    DBuf<BatchArgs> d_units[kRing];
+    DBuf<uint32_t> d_idx[kRing];
+    hipEvent_t ring_ev[kRing] = {};
+    bool ring_used[kRing] = {};
+    uint32_t cap_units = 0;
+    int ring = 0;
+    uint64_t rounds = 0, launches = 0, units_total = 0;
+    ~tsg_batch() {
+        for (int r = 0; r < kRing; ++r) {
+            if (h_units[r]) (void)hipHostFree(h_units[r]);
+            if (h_idx[r]) (void)hipHostFree(h_idx[r]);
+            if (ring_ev[r]) (void)hipEventDestroy(ring_ev[r]);
+        }
+    }
+};
+
+namespace {
+
+int batch_admit(tsg_batch* b) {
+    std::vector<tsg_job*> adm;
+    while (b->active.size() < b->max_active && !b->pending.empty()) {
+        tsg_batch::Entry e = b->pending.front();
+        b->pending.erase(b->pending.begin());
+        b->active.push_back(e);
+        adm.push_back(e.job);
+    }
+    if (adm.empty()) return TSG_OK;
+    // pick orders of the admitted jobs, one host thread per job (<= 16 at a time)
+    std::atomic<size_t> next{0};
+    unsigned nt = std::min<size_t>(adm.size(), 16);
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t)
+        th.emplace_back([&] {
+            for (size_t q; (q = next.fetch_add(1)) < adm.size();) precompute_picks(adm[q]);
+        });
+    for (auto& x : th) x.join();
+    for (tsg_job* j : adm) {
+        int rc = step_begin(j);
+        if (rc) return rc;
+    }
+    return TSG_OK;
+}
+
+// One round: every active job contributes its next batch.
+int batch_round(tsg_batch* b) {
+    hipStream_t s = b->ctx->stream;
+    int rc;
+    if ((rc = batch_admit(b))) return rc;
+    if (b->active.empty()) return TSG_OK;
+    struct U {
+        tsg_job* j;
+        uint64_t i0, i1;
+    };
+    std::vector<U> units[2];
+    for (size_t q = 0; q < b->active.size();) {
+        tsg_job* j = b->active[q].job;
+        uint64_t i0 = 0, i1 = 0;
+        bool got = false;
+        while (!step_done(j)) {
+            if (!j->step.open && (rc = step_open_stage(j))) return rc;
+            if (step_peek(j, i0, i1)) {
+                got = true;
+                break;
+            }
+            if ((rc = step_close_stage(j))) return rc;
+        }
+        if (!got) {
+            if ((rc = step_end(j))) return rc;
+            b->done.push_back(b->active[q]);
+            b->active.erase(b->active.begin() + q);
+            continue;
+        }
+        if ((rc = step_ensure_rc(j, i0, i1))) return rc;
+        units[j->guided ? 1 : 0].push_back({j, i0, i1});
+        ++q;
+    }
+    const uint32_t nu = (uint32_t)(units[0].size() + units[1].size());
+    if (nu == 0) return TSG_OK;
+    // ring slot: wait until the round that last used it has finished
+    const int r = b->ring;
+    b->ring = (b->ring + 1) % tsg_batch::kRing;
+    if (b->ring_used[r]) HIP_TRY(hipEventSynchronize(b->ring_ev[r]));
+    if (nu > b->cap_units) {
+        uint32_t cap = std::max<uint32_t>(nu, 2 * b->cap_units);
+        HIP_TRY(hipDeviceSynchronize());
+        for (int q = 0; q < tsg_batch::kRing; ++q) {
+            if (b->h_units[q]) HIP_TRY(hipHostFree(b->h_units[q]));
+            if (b->h_idx[q]) HIP_TRY(hipHostFree(b->h_idx[q]));
+            HIP_TRY(hipHostMalloc((void**)&b->h_units[q], cap * sizeof(BatchArgs)));
+            HIP_TRY(hipHostMalloc((void**)&b->h_idx[q], 4 * (cap + 2) * sizeof(uint32_t)));
+            HIP_TRY(b->d_units[q].alloc(cap));
+            HIP_TRY(b->d_idx[q].alloc(4 * (cap + 2)));
+        }
+        b->cap_units = cap;
+    }
+    BatchArgs* hu = b->h_units[r];
+    uint32_t* hi = b->h_idx[r];
+    // layout: kind 0 units then kind 1 units; idx = wg starts (cap+1) then item starts (cap+1), per kind
+    uint32_t off = 0, n_wg[2] = {0, 0}, n_items[2] = {0, 0};
+    uint32_t* wg_s[2] = {hi, hi + (b->cap_units + 1)};
+    uint32_t* it_s[2] = {hi + 2 * (b->cap_units + 1), hi + 3 * (b->cap_units + 1)};
+    // one team size for the round: the units share the device
+    uint64_t round_items = 0;
+    for (int kd = 0; kd < 2; ++kd)
+        for (const U& u : units[kd]) round_items += u.i1 - u.i0;
+    const uint32_t team = team_for(round_items);
+    const uint32_t ipw = items_per_wg(team);
+    for (int kd = 0; kd < 2; ++kd) {
+        for (size_t q = 0; q < units[kd].size(); ++q) {
+            const U& u = units[kd][q];
+            BatchArgs a = u.j->step.a;
+            a.n_res = (uint32_t)u.j->n_res;
+            a.item0 = (uint32_t)u.i0;
+            a.item1 = (uint32_t)u.i1;
+            a.fused_commit = 0;
+            a.team = team;
+            a.dbg_clk = nullptr;
+            hu[off + q] = a;
+            wg_s[kd][q] = n_wg[kd];
+            it_s[kd][q] = n_items[kd];
+            n_wg[kd] += (uint32_t)((u.i1 - u.i0 + ipw - 1) / ipw);
+            n_items[kd] += (uint32_t)(u.i1 - u.i0);
+        }
+        off += (uint32_t)units[kd].size();
+    }
+    HIP_TRY(hipMemcpyAsync(b->d_units[r].p, hu, nu * sizeof(BatchArgs), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(b->d_idx[r].p, hi, 4 * (b->cap_units + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    const BatchArgs* du[2] = {b->d_units[r].p, b->d_units[r].p + units[0].size()};
+    const uint32_t* dw[2] = {b->d_idx[r].p, b->d_idx[r].p + (b->cap_units + 1)};
+    const uint32_t* di[2] = {b->d_idx[r].p + 2 * (b->cap_units + 1), b->d_idx[r].p + 3 * (b->cap_units + 1)};
+    for (int kd = 0; kd < 2; ++kd) {
+        if (units[kd].empty()) continue;
+        HIP_TRY(launch_resolve_multi(du[kd], dw[kd], (uint32_t)units[kd].size(), n_wg[kd], kd == 1, s));
+        HIP_TRY(launch_commit_multi(du[kd], di[kd], (uint32_t)units[kd].size(), n_items[kd], s));
+        b->launches += 2;
+    }
+    HIP_TRY(hipEventRecord(b->ring_ev[r], s));
+    b->ring_used[r] = true;
+    for (int kd = 0; kd < 2; ++kd)
+        for (const U& u : units[kd]) step_advance(u.j, u.i0, u.i1);
+    b->rounds += 1;
+    b->units_total += nu;
+    return TSG_OK;
+}
+
+// Finalise jobs whose last launch has completed.
+int batch_reap(tsg_batch* b, bool wait) {
+    for (size_t q = 0; q < b->done.size();) {
+        tsg_job* j = b->done[q].job;
+        if (!wait && hipEventQuery(j->ev_run1) == hipErrorNotReady) {
+            ++q;
+            continue;
+        }
+        b->done[q].status = step_finish(j);
+        b->taken.push_back(b->done[q]);
+        b->done.erase(b->done.begin() + q);
+    }
     return TSG_OK;
 }
 
@@ -927,7 +1230,6 @@ int tsg_job_set_schedule(tsg_job* job, const tsg_schedule* s) {
 
 int tsg_job_run(tsg_job* job, tsg_progress_fn progress, void* user) {
     if (!job) return fail(TSG_ERR_INVALID_ARGUMENT, "null job");
-    if (job->ran) return fail(TSG_ERR_INVALID_ARGUMENT, "a job runs once (Session::run consumes the session)");
     HIP_TRY(hipSetDevice(job->ctx->device));
     std::lock_guard<std::mutex> lk(job->ctx->mu);
     return run_job(job, progress, user);
@@ -1162,7 +1464,8 @@ int tsg_score_batch(tsg_ctx* ctx, uint32_t w, uint32_t h, const uint8_t* color, 
                     const tsg_image* target_guide, const float* lut_color, const float* lut_guide, uint32_t k,
                     uint32_t cmax, uint32_t n, const uint32_t* flat, const uint32_t* nbr_count, const int32_t* nbrs,
                     const uint32_t* cand_count, const int32_t* cands, float* full_cost, uint32_t* best,
-                    float* best_score, uint32_t* best_pruned, float* best_score_pruned) {
+                    float* best_score, uint32_t* best_pruned, float* best_score_pruned, uint32_t team) {
+    if (team > 8 || (team & (team - 1)) != 0) return fail(TSG_ERR_INVALID_ARGUMENT, "team must be 0, 1, 2, 4 or 8");
     if (!ctx || !color || !coord || !ids || !maps || !lut_color || !flat || !nbr_count || !nbrs || !cand_count ||
         !cands || !full_cost || !best || !best_score)
         return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
@@ -1299,6 +1602,7 @@ int tsg_score_batch(tsg_ctx* ctx, uint32_t w, uint32_t h, const uint8_t* color, 
     if (best_pruned || best_score_pruned) {
         // production evaluation (dedup + pruning) of the same lists
         a.ext_full_cost = nullptr;
+        a.team = team;
         HIP_TRY(launch_resolve(a, guided, s));
         HIP_TRY(hipStreamSynchronize(s));
         HIP_TRY(hipMemcpy(res.data(), dres.p, n * sizeof(uint4), hipMemcpyDeviceToHost));
@@ -1310,6 +1614,145 @@ int tsg_score_batch(tsg_ctx* ctx, uint32_t w, uint32_t h, const uint8_t* color, 
         }
     }
     return TSG_OK;
+}
+
+
+int tsg_job_footprint(const tsg_job* job, uint64_t* bytes) {
+    if (!job || !bytes) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    *bytes = job_footprint(job);
+    return TSG_OK;
+}
+
+int tsg_footprint_estimate(const tsg_params* prm, uint32_t n_examples, const uint32_t* example_wh,
+                           const int* sample_kind, const uint32_t* guide_wh, uint32_t out_w, uint32_t out_h,
+                           uint64_t* bytes) {
+    if (!prm || !example_wh || !bytes) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    const uint64_t levels = prm->backtrack_stages == 0 ? 1 : prm->backtrack_stages;
+    const uint64_t N = (uint64_t)out_w * out_h;
+    const uint64_t P = prm->backtrack_stages;
+    const uint64_t m = std::max<uint64_t>(1, prm->random_sample_locations);
+    uint64_t b = 0, nf = 0;
+    for (uint32_t i = 0; i < n_examples; ++i) {
+        const uint64_t px = (uint64_t)example_wh[2 * i] * example_wh[2 * i + 1];
+        const int kd = sample_kind ? sample_kind[i] : TSG_SAMPLE_ALL;
+        b += levels * px * 4 + (kd == TSG_SAMPLE_IMAGE ? px : 0);
+        if (guide_wh) b += levels * (uint64_t)guide_wh[2 * i] * guide_wh[2 * i + 1] * 4;
+        nf += kd != TSG_SAMPLE_IGNORE;
+    }
+    if (guide_wh) b += levels * N * 4;
+    b += levels * nf * sizeof(MapDesc);
+    // state, score, resolved list, per-item scores, resolved-mask tiles
+    b += N * (sizeof(uint4) + 12) + (uint64_t)((out_w + 7) / 8) * ((out_h + 7) / 8) * 8;
+    // scratch under the engine: largest batch results, one random-candidate
+    // chunk, the small-batch tables, per-stage statistics
+    // (default schedule: largest batch N >> 5, small batches for 17 << 6 items per segment)
+    const uint64_t maxb = std::min<uint64_t>(N, std::max<uint64_t>((N >> 5) + 1, small_batch_max()));
+    b += maxb * sizeof(uint4);
+    b += std::min<uint64_t>(N, std::max<uint64_t>(maxb, kEngineRcBudget / (8 * m))) * m * sizeof(uint2);
+    uint64_t items = 0;
+    for (uint64_t t = 0; t <= P; ++t) {
+        float v = std::pow(prm->backtrack_percent, (float)t) * (float)N;
+        items += v > 0.0f ? (uint64_t)v : 0;
+    }
+    const uint64_t per_seg = (uint64_t)(small_batch_max() + 1) << 6;
+    b += (3 * std::min<uint64_t>(items, 2 * (P + 1) * per_seg) + 4 * (P + 1) + 16) * 4;
+    b += (P + 1) * 16 + 512 * 4 + 64;
+    *bytes = b;
+    return TSG_OK;
+}
+
+int tsg_batch_create(tsg_ctx* ctx, uint32_t max_active, tsg_batch** out) {
+    if (!ctx || !out) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    if (max_active == 0) return fail(TSG_ERR_INVALID_ARGUMENT, "max_active must be positive");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::unique_ptr<tsg_batch> b(new tsg_batch());
+    b->ctx = ctx;
+    b->max_active = max_active;
+    for (int r = 0; r < tsg_batch::kRing; ++r) HIP_TRY(hipEventCreateWithFlags(&b->ring_ev[r], hipEventDisableTiming));
+    *out = b.release();
+    return TSG_OK;
+}
+
+int tsg_batch_submit(tsg_batch* b, tsg_job* job, uint64_t* ticket) {
+    if (!b || !job || !ticket) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    if (job->ctx != b->ctx) return fail(TSG_ERR_INVALID_ARGUMENT, "job belongs to another context");
+    if (job->ran || job->step.active) return fail(TSG_ERR_INVALID_ARGUMENT, "a job runs once");
+    job->rc_budget = kEngineRcBudget;
+    *ticket = b->next_ticket++;
+    b->pending.push_back({*ticket, job, 0});
+    return TSG_OK;
+}
+
+int tsg_batch_step(tsg_batch* b, uint32_t max_rounds, uint32_t* in_flight) {
+    if (!b) return fail(TSG_ERR_INVALID_ARGUMENT, "null batch");
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    std::lock_guard<std::mutex> lk(b->ctx->mu);
+    for (uint32_t r = 0; r < max_rounds && !(b->active.empty() && b->pending.empty()); ++r) {
+        int rc = batch_round(b);
+        if (rc) return rc;
+    }
+    int rc = batch_reap(b, false);
+    if (rc) return rc;
+    if (in_flight) *in_flight = (uint32_t)(b->active.size() + b->pending.size() + b->done.size());
+    return TSG_OK;
+}
+
+int tsg_batch_run_all(tsg_batch* b) {
+    if (!b) return fail(TSG_ERR_INVALID_ARGUMENT, "null batch");
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    std::lock_guard<std::mutex> lk(b->ctx->mu);
+    while (!(b->active.empty() && b->pending.empty())) {
+        int rc = batch_round(b);
+        if (rc) return rc;
+    }
+    return batch_reap(b, true);
+}
+
+int tsg_batch_poll(tsg_batch* b, uint64_t* tickets, int* status, uint32_t cap, uint32_t* n) {
+    if (!b || !n) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    {
+        std::lock_guard<std::mutex> lk(b->ctx->mu);
+        int rc = batch_reap(b, false);
+        if (rc) return rc;
+    }
+    uint32_t c = 0;
+    for (const tsg_batch::Entry& e : b->taken) {
+        if (c >= cap) break;
+        if (tickets) tickets[c] = e.ticket;
+        if (status) status[c] = e.status;
+        ++c;
+    }
+    *n = c;
+    return TSG_OK;
+}
+
+int tsg_batch_take(tsg_batch* b, uint64_t ticket, tsg_job** job) {
+    if (!b || !job) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    for (size_t q = 0; q < b->taken.size(); ++q)
+        if (b->taken[q].ticket == ticket) {
+            *job = b->taken[q].job;
+            b->taken.erase(b->taken.begin() + q);
+            return TSG_OK;
+        }
+    return fail(TSG_ERR_INVALID_ARGUMENT, "ticket not finished or unknown");
+}
+
+int tsg_batch_stats(tsg_batch* b, uint64_t* rounds, uint64_t* launches, uint64_t* units) {
+    if (!b) return fail(TSG_ERR_INVALID_ARGUMENT, "null batch");
+    if (rounds) *rounds = b->rounds;
+    if (launches) *launches = b->launches;
+    if (units) *units = b->units_total;
+    return TSG_OK;
+}
+
+void tsg_batch_destroy(tsg_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    for (auto* v : {&b->active, &b->pending, &b->done, &b->taken})
+        for (auto& e : *v) tsg_job_destroy(e.job);
+    delete b;
 }
 
 }  // extern "C"

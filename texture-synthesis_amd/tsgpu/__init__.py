@@ -110,7 +110,17 @@ def lib():
     L.tsg_knn_batch.argtypes = [vp, vp, u32, u32, u32, vp, u32, vp, vp]
     L.tsg_random_candidates.argtypes = [vp, u32, P(Image), P(Image), u64, u32, u32, u32, vp]
     L.tsg_score_batch.argtypes = [vp, u32, u32, vp, vp, vp, u32, P(Image), P(Image), P(Image), vp, vp, u32, u32,
-                                  u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+                                  u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u32]
+    L.tsg_job_footprint.argtypes = [vp, P(u64)]
+    L.tsg_footprint_estimate.argtypes = [P(Params), u32, vp, vp, vp, u32, u32, P(u64)]
+    L.tsg_batch_create.argtypes = [vp, u32, P(vp)]
+    L.tsg_batch_submit.argtypes = [vp, vp, P(u64)]
+    L.tsg_batch_step.argtypes = [vp, u32, P(u32)]
+    L.tsg_batch_run_all.argtypes = [vp]
+    L.tsg_batch_poll.argtypes = [vp, vp, vp, u32, P(u32)]
+    L.tsg_batch_take.argtypes = [vp, u64, P(vp)]
+    L.tsg_batch_stats.argtypes = [vp, P(u64), P(u64), P(u64)]
+    L.tsg_batch_destroy.argtypes = [vp]
     _lib = L
     return L
 
@@ -335,45 +345,136 @@ class Session:
             cb = PROGRESS_FN()
         try:
             _check(L.tsg_job_run(self._job, cb, None))
-            w, h = self._out.width, self._out.height
-            img = np.zeros((h, w, 4), np.uint8)
-            coord = np.zeros((h * w, 3), np.uint32)
-            ids = np.zeros((h * w, 2), np.uint32)
-            rf = np.zeros(h * w, np.uint32)
-            rs = np.zeros(h * w, np.float32)
-            _check(L.tsg_job_download(self._job, _ptr(img), _ptr(coord), _ptr(ids), _ptr(rf), _ptr(rs)))
-            n = 16
-            sums = np.zeros(n, np.float64)
-            cnts = np.zeros(n, np.uint64)
-            items = np.zeros(n, np.uint64)
-            got = L.tsg_job_stage_stats(self._job, _ptr(sums), _ptr(cnts), _ptr(items), n)
-            if got < 0:
-                _check(got)
-            ms = C.c_double()
-            nb = C.c_uint64()
-            _check(L.tsg_job_timing(self._job, C.byref(ms), C.byref(nb)))
-            kst = None
-            if getattr(self, "_profiled", False):
-                kms = np.zeros(5, np.float64)
-                kit = np.zeros(5, np.uint64)
-                kla = np.zeros(5, np.uint64)
-                _check(L.tsg_job_kernel_stats(self._job, _ptr(kms), _ptr(kit), _ptr(kla)))
-                kst = {n: dict(ms=float(kms[i]), items=int(kit[i]), launches=int(kla[i]))
-                       for i, n in enumerate(GeneratedImage.KERNELS)}
-                if getattr(self, "_profiled", 0) >= 2:
-                    clk = np.zeros((h * w, 16), np.uint64)
-                    _check(L.tsg_job_phase_clocks(self._job, _ptr(clk), h * w))
-                    kst["phase_clocks"] = clk
-            return GeneratedImage(img, coord, ids, rf, rs, (sums[:got], cnts[:got], items[:got]), ms.value,
-                                  nb.value, self._original_maps, kst)
+            return self._collect()
         finally:
             L.tsg_job_destroy(self._job)
             self._job = None
+
+    def footprint(self) -> int:
+        """Device bytes the job holds now (inputs, state and scratch)."""
+        v = C.c_uint64()
+        _check(lib().tsg_job_footprint(self._job, C.byref(v)))
+        return v.value
+
+    def _collect(self) -> GeneratedImage:
+        """Downloads the finished job's maps, statistics and timings."""
+        L = lib()
+        w, h = self._out.width, self._out.height
+        img = np.zeros((h, w, 4), np.uint8)
+        coord = np.zeros((h * w, 3), np.uint32)
+        ids = np.zeros((h * w, 2), np.uint32)
+        rf = np.zeros(h * w, np.uint32)
+        rs = np.zeros(h * w, np.float32)
+        _check(L.tsg_job_download(self._job, _ptr(img), _ptr(coord), _ptr(ids), _ptr(rf), _ptr(rs)))
+        n = 16
+        sums = np.zeros(n, np.float64)
+        cnts = np.zeros(n, np.uint64)
+        items = np.zeros(n, np.uint64)
+        got = L.tsg_job_stage_stats(self._job, _ptr(sums), _ptr(cnts), _ptr(items), n)
+        if got < 0:
+            _check(got)
+        ms = C.c_double()
+        nb = C.c_uint64()
+        _check(L.tsg_job_timing(self._job, C.byref(ms), C.byref(nb)))
+        kst = None
+        if getattr(self, "_profiled", False):
+            kms = np.zeros(5, np.float64)
+            kit = np.zeros(5, np.uint64)
+            kla = np.zeros(5, np.uint64)
+            _check(L.tsg_job_kernel_stats(self._job, _ptr(kms), _ptr(kit), _ptr(kla)))
+            kst = {n: dict(ms=float(kms[i]), items=int(kit[i]), launches=int(kla[i]))
+                   for i, n in enumerate(GeneratedImage.KERNELS)}
+            if getattr(self, "_profiled", 0) >= 2:
+                clk = np.zeros((h * w, 16), np.uint64)
+                _check(L.tsg_job_phase_clocks(self._job, _ptr(clk), h * w))
+                kst["phase_clocks"] = clk
+        gen = GeneratedImage(img, coord, ids, rf, rs, (sums[:got], cnts[:got], items[:got]), ms.value,
+                             nb.value, self._original_maps, kst)
+        gen.device_bytes = self.footprint()  # every buffer the run grew
+        return gen
 
     def __del__(self):
         if getattr(self, "_job", None) and _lib is not None:
             _lib.tsg_job_destroy(self._job)
             self._job = None
+
+
+class Batch:
+    """Many sessions in flight on one device.
+
+    Each round runs the next Jacobi batch of every active session in one
+    resolve launch and one commit launch; a session's result is identical to
+    Session.run on it alone. No counterpart in the reference, whose
+    Session::run (session.rs:37-66) synthesises one image at a time."""
+
+    def __init__(self, ctx: Optional[Context] = None, max_active: int = 256):
+        self._ctx = ctx or Context.default()
+        h = C.c_void_p()
+        _check(lib().tsg_batch_create(self._ctx._h, int(max_active), C.byref(h)))
+        self._h = h
+        self._sessions = {}
+        self._status = {}
+
+    def submit(self, session: Session) -> int:
+        """Hands the session over; returns its ticket."""
+        if session._job is None:
+            raise RuntimeError("a session runs once")
+        t = C.c_uint64()
+        _check(lib().tsg_batch_submit(self._h, session._job, C.byref(t)))
+        self._sessions[t.value] = session
+        session._job = None
+        return t.value
+
+    def step(self, max_rounds: int = 1) -> int:
+        """Runs up to max_rounds rounds; returns the sessions not yet finished."""
+        n = C.c_uint32()
+        _check(lib().tsg_batch_step(self._h, int(max_rounds), C.byref(n)))
+        return n.value
+
+    def run_all(self):
+        _check(lib().tsg_batch_run_all(self._h))
+
+    def poll(self):
+        """[(ticket, status)] of finished sessions not yet taken."""
+        cap = max(1, len(self._sessions))
+        tk = np.zeros(cap, np.uint64)
+        st = np.zeros(cap, np.int32)
+        n = C.c_uint32()
+        _check(lib().tsg_batch_poll(self._h, _ptr(tk), _ptr(st), cap, C.byref(n)))
+        out = [(int(tk[i]), int(st[i])) for i in range(n.value)]
+        self._status.update(out)
+        return out
+
+    def take(self, ticket: int) -> GeneratedImage:
+        """Result of a finished session; raises the session's run error, if any."""
+        if ticket not in self._status:
+            self.poll()
+        status = self._status.pop(ticket, None)
+        job = C.c_void_p()
+        _check(lib().tsg_batch_take(self._h, ticket, C.byref(job)))
+        s = self._sessions.pop(ticket)
+        s._job = job
+        try:
+            if status:
+                raise TsgError(status, "session failed in the batch")
+            return s._collect()
+        finally:
+            lib().tsg_job_destroy(job)
+            s._job = None
+
+    def stats(self):
+        r, la, u = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(lib().tsg_batch_stats(self._h, C.byref(r), C.byref(la), C.byref(u)))
+        return dict(rounds=r.value, launches=la.value, units=u.value)
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.tsg_batch_destroy(self._h)
+            self._h = None
+            self._sessions.clear()
+
+    def __del__(self):
+        self.close()
 
 
 class SessionBuilder:
@@ -466,6 +567,24 @@ class SessionBuilder:
         raise TsgError(-11, "resizing inputs (load_image with CatmullRom) is the caller's step; "
                             "pass images at their final size")
 
+    def _out_dims(self) -> Dims:
+        return self.inpaint[2] if self.inpaint is not None else self._output_size
+
+    def footprint_estimate(self) -> int:
+        """Device bytes the built session holds while it runs under a Batch."""
+        out = self._out_dims()
+        wh = np.array([[e.img.shape[1], e.img.shape[0]] for e in self.examples], np.uint32).reshape(-1)
+        kinds = np.array([TSG_SAMPLE_IMAGE if not isinstance(e.sample_method, str) else
+                          TSG_SAMPLE_IGNORE if e.sample_method == SampleMethod.Ignore else TSG_SAMPLE_ALL
+                          for e in self.examples], np.int32)
+        gwh = None
+        if self.target_guide is not None and self.examples and all(e.guide is not None for e in self.examples):
+            gwh = np.array([[e.guide.shape[1], e.guide.shape[0]] for e in self.examples], np.uint32).reshape(-1)
+        v = C.c_uint64()
+        _check(lib().tsg_footprint_estimate(C.byref(self.params), len(self.examples), _ptr(wh), _ptr(kinds),
+                                            _ptr(gwh), out.width, out.height, C.byref(v)))
+        return v.value
+
     def build(self) -> Session:
         if self.inpaint is not None:
             mask, idx, dims = self.inpaint
@@ -536,7 +655,7 @@ def random_candidates(maps, sample_masks, stage_seed: int, item0: int, n: int, m
 
 
 def score_batch(color, coord, ids, maps, lut_color, k, flat, nbr_count, nbrs, cand_count, cands, guides=None,
-                target_guide=None, lut_guide=None, ctx: Optional[Context] = None):
+                target_guide=None, lut_guide=None, ctx: Optional[Context] = None, team: int = 1):
     ctx = ctx or Context.default()
     h, w = color.shape[:2]
     color = np.ascontiguousarray(color, np.uint8)
@@ -561,5 +680,5 @@ def score_batch(color, coord, ids, maps, lut_color, k, flat, nbr_count, nbrs, ca
     _check(lib().tsg_score_batch(ctx._h, w, h, _ptr(color), _ptr(coord), _ptr(ids), len(maps), M.arr,
                                  G.arr if G else None, T.arr if T else None, _ptr(lc), _ptr(lg), k, cmax, n,
                                  _ptr(flat), _ptr(nbr_count), _ptr(nbrs), _ptr(cand_count), _ptr(cands),
-                                 _ptr(full), _ptr(best), _ptr(bs), _ptr(bp), _ptr(bsp)))
+                                 _ptr(full), _ptr(best), _ptr(bs), _ptr(bp), _ptr(bsp), int(team)))
     return full, best, bs, bp, bsp
