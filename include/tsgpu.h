@@ -68,6 +68,31 @@ typedef struct tsg_schedule {
     uint32_t max_batch;
 } tsg_schedule;
 
+/* Launch structure of the device path. Results never depend on it (every
+ * choice resolves the same batches against the same frozen states); it
+ * trades launch count, latency and occupancy, and is what the autotuner
+ * picks.
+ *   small_max    batches of at most this many items run in the one-workgroup
+ *                loop kernel (0..16; 0 = never)
+ *   fuse_max     larger batches of at most this many items commit in their
+ *                resolve launch's last workgroup (0 = separate commit launch)
+ *   team_max     waves per item of latency-bound batches (1, 2, 4 or 8)
+ *   team_waves   resident waves the team size is sized against
+ *   prefetch_max batches of at most this many items get their neighbour
+ *                lists computed ahead, window by window (0 = never)
+ *   ahead        1: a single-job run prepares each stage's content-independent
+ *                work (random candidates, pick ranks, every neighbour list)
+ *                during the previous stage on a second stream, when it fits
+ *                in device memory; 0: per window on the job stream */
+typedef struct tsg_tuning {
+    uint32_t small_max;
+    uint32_t fuse_max;
+    uint32_t team_max;
+    uint32_t team_waves;
+    uint32_t prefetch_max;
+    uint32_t ahead;
+} tsg_tuning;
+
 typedef struct tsg_image {
     uint32_t w, h;
     const uint8_t* rgba; /* w*h*4 bytes */
@@ -87,6 +112,7 @@ typedef void (*tsg_progress_fn)(void* user, const uint8_t* rgba, uint32_t w, uin
 
 void tsg_params_default(tsg_params* p);
 void tsg_schedule_default(tsg_schedule* s);
+void tsg_tuning_default(tsg_tuning* t);
 const char* tsg_last_error(void);
 const char* tsg_version(void);
 
@@ -112,6 +138,8 @@ int tsg_job_create(tsg_ctx* ctx, const tsg_params* params, uint32_t n_examples, 
                    uint32_t out_w, uint32_t out_h, tsg_job** out);
 /* Overrides the default Jacobi schedule (before tsg_job_run). */
 int tsg_job_set_schedule(tsg_job* job, const tsg_schedule* s);
+/* Overrides the default launch structure (before tsg_job_run). */
+int tsg_job_set_tuning(tsg_job* job, const tsg_tuning* t);
 /* Session::run (session.rs:37-66): random init, then every stage. */
 int tsg_job_run(tsg_job* job, tsg_progress_fn progress, void* user);
 /* GeneratedImage accessors (lib.rs:378-455, ms.rs:605-684). Any pointer may
@@ -134,12 +162,12 @@ int tsg_job_timing(tsg_job* job, double* run_ms, uint64_t* n_batches);
 int tsg_job_set_profiling(tsg_job* job, int level);
 /* Phase record of the first n_items work items of the first stage (16 u64
  * each) after a level-2 profiled run: [0..6] the phase timestamps above,
- * [7] kNN search done before its final selection, [8] ring-search steps,
+ * [7] kNN search done (in-kernel searches only), [8] ring-search steps,
  * [9] keys buffered before the final selection. */
 int tsg_job_phase_clocks(tsg_job* job, uint64_t* out, uint64_t n_items);
 /* After a profiled run, per kernel [resolve, commit, random candidates,
- * recolour, stats]: summed milliseconds, items processed, launches.
- * Each array holds 5 entries. */
+ * recolour, stats, kNN prefetch]: summed milliseconds, items processed,
+ * launches. Each array holds 6 entries. */
 int tsg_job_kernel_stats(tsg_job* job, double* ms, uint64_t* items, uint64_t* launches);
 void tsg_job_destroy(tsg_job* job);
 

@@ -230,6 +230,23 @@ def test_default_schedule_bit_exact_c1(ctx, seed):
     assert_same(gen, job)
 
 
+@pytest.mark.parametrize("tuning", [
+    dict(small_max=0, fuse_max=0, team_max=1, prefetch_max=0, ahead=0),
+    dict(small_max=4, fuse_max=64, team_max=2, prefetch_max=64, ahead=0),
+    dict(small_max=16, fuse_max=100000, team_max=8, team_waves=1 << 20, prefetch_max=100000, ahead=0),
+    dict(small_max=16, fuse_max=2048, team_max=4, ahead=1),
+])
+def test_launch_tuning_does_not_change_results(ctx, tuning):
+    # every launch structure resolves the same batches against the same states
+    ex = synth.example(211, 128, 128)
+    prm = R.default_params(seed=211)
+    job = R.Job(prm, [ex], 256, 256)
+    job.run(threads=8, **DEFAULT)
+    sess = ts.Session.builder().add_example(ex).seed(211).output_size(ts.Dims.square(256)).build()
+    sess.set_schedule(**DEFAULT).set_tuning(**tuning)
+    assert_same(sess.run(), job)
+
+
 def test_multi_example_random_init_bit_exact(ctx):
     exs = [synth.example(40 + i, 50 + 10 * i, 50) for i in range(4)]
     gen, job = run_pair(exs, 96, 80, seed=211, random_init=10, nearest_neighbors=20)

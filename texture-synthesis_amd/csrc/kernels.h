@@ -54,6 +54,32 @@ struct BatchArgs {
     float* score_w;
     unsigned long long* dbg_clk;  // [stage item][16] phase timestamps (profiling)
     uint32_t team;                // waves per item (1, 2, 4, 8; 0 = 1)
+    // neighbour lists computed ahead for stage items [pre_base, pre_end)
+    const int32_t* pre_nbrs;      // [item - pre_base][k] packed dx | dy << 16
+    const uint32_t* pre_cnt;
+    uint32_t pre_base, pre_end;
+};
+
+// kNN of stage items [item0, item1) ahead of their batches: item i sees the
+// resolved set at the start of its batch, i.e. every pixel of `mask` (one
+// u64 per 8x8 tile) and the pixels p of `mask_new` with rank[p] <
+// n_at[i - item0] (rank = position in the resolved list). tile_min: least
+// rank of each tile's mask_new pixels. mask_new may be null.
+struct PrefetchArgs {
+    uint32_t W, H, tiles_x, tiles_y;
+    const uint64_t* mask;
+    const uint64_t* mask_new;
+    const uint32_t* tile_min;
+    const uint32_t* rank;
+    const uint32_t* rlist;
+    const uint32_t* work;
+    const uint32_t* n_at;
+    const int32_t* offtab;
+    uint32_t offtab_n;
+    uint32_t k;
+    uint32_t item0, item1;
+    int32_t* out_nbrs;            // [item - item0][k]
+    uint32_t* out_cnt;
 };
 
 struct RandArgs {
@@ -85,6 +111,22 @@ struct RecolorArgs {
     uint4* state;
 };
 
+// Result maps of a finished job from its packed state: rgba (u32 per pixel),
+// coord (x, y, map), ids (patch, map); resolved list order and the score of
+// each listed pixel. Null outputs are skipped.
+struct UnpackArgs {
+    const uint4* state;
+    uint32_t n;
+    uint32_t* rgba;
+    uint32_t* coord;
+    uint32_t* ids;
+    const uint32_t* rlist;
+    const float* score;
+    uint32_t n_res;
+    float* rscore;
+};
+hipError_t launch_unpack(const UnpackArgs& a, hipStream_t s);
+
 // per-stage statistics: sum and count of non-negative item scores
 struct StatsArgs {
     const float* item_score;
@@ -93,6 +135,17 @@ struct StatsArgs {
     unsigned long long* count;
 };
 
+hipError_t launch_knn_prefetch(const PrefetchArgs& a, hipStream_t s);
+// random candidates of several jobs: unit u's items start at item_start[u]
+// of the launch's `total` threads (item_start[0] = 0)
+hipError_t launch_rand_multi(const RandArgs* units, const uint32_t* item_start, uint32_t n_units, uint32_t total,
+                             hipStream_t s);
+// rank[rlist[q]] = q for q in [lo, hi)
+hipError_t launch_rank(const uint32_t* rlist, uint32_t lo, uint32_t hi, uint32_t* rank, hipStream_t s);
+// sets the mask bits of rlist[lo, hi); with tile_min, also the least q per tile
+hipError_t launch_mask_or(const uint32_t* rlist, uint32_t lo, uint32_t hi, uint32_t W, uint32_t tiles_x,
+                          uint64_t* mask, uint32_t* tile_min, hipStream_t s);
+
 hipError_t configure_kernels();
 size_t resolve_lds_bytes(bool guided);
 size_t resolve_small_lds_bytes(bool guided);
@@ -100,7 +153,7 @@ uint32_t small_batch_max();
 // items per workgroup of k_resolve / k_resolve_multi with `team` waves per item
 uint32_t items_per_wg(uint32_t team);
 // waves per item for a batch of n items (1, 2, 4 or 8)
-uint32_t team_for(uint64_t n);
+uint32_t team_for(uint64_t n, uint32_t team_max, uint32_t waves);
 // one round of the multi-job engine: units[u] is a job's batch, its workgroups
 // start at wg_start[u] (n_units + 1 entries), its items at item_start[u]
 hipError_t launch_resolve_multi(const BatchArgs* units, const uint32_t* wg_start, uint32_t n_units, uint32_t n_wg,

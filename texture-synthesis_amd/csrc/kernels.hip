@@ -200,8 +200,50 @@ __device__ __forceinline__ bool px_set(const uint64_t* mask, uint32_t tiles_x, i
     return (wd >> (((Y & 7) << 3) | (X & 7))) & 1ull;
 }
 
+// What the kNN routes read: output size, resolved-pixel mask (one u64 per
+// 8x8 tile), resolved list, offset table. With `mask_new`, a pixel set there
+// also counts when rank[p] < n_at (resolved before an earlier moment);
+// tile_min[t] is the least rank among tile t's mask_new pixels.
+struct KnnView {
+    uint32_t W, H, tiles_x, tiles_y;
+    const uint64_t* mask;
+    const uint32_t* rlist;
+    const int32_t* offtab;
+    uint32_t offtab_n;
+    const uint64_t* mask_new;
+    const uint32_t* tile_min;
+    const uint32_t* rank;
+    uint32_t n_at;
+};
+
+// The pixels of tile (gx, gy)'s new-pixel word nw that were resolved before
+// v.n_at; rank loads are issued eight at a time.
+__device__ __forceinline__ uint64_t filter_new(const KnnView& v, int gx, int gy, uint64_t nw) {
+    uint64_t keep = 0;
+    while (nw) {
+        int b[8];
+        uint32_t r[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            b[q] = -1;
+            if (nw) {
+                b[q] = __ffsll((long long)nw) - 1;
+                nw &= nw - 1;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            r[q] = b[q] >= 0 ? v.rank[(uint32_t)(gy * 8 + (b[q] >> 3)) * v.W + (uint32_t)(gx * 8 + (b[q] & 7))]
+                             : 0xffffffffu;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (b[q] >= 0 && r[q] < v.n_at) keep |= 1ull << b[q];
+    }
+    return keep;
+}
+
 // Exhaustive scan of the resolved list (sparse phase).
-__device__ __forceinline__ void knn_list(const BatchArgs& a, WaveLds* L, uint32_t n_res, int x, int y, uint32_t k, int lane, Sel& s) {
+__device__ __forceinline__ void knn_list(const KnnView& a, WaveLds* L, uint32_t n_res, int x, int y, uint32_t k, int lane, Sel& s) {
     // four chunks of the list per memory round trip
     for (uint32_t base = 0; base < n_res; base += 4 * kWave) {
         uint32_t f[4];
@@ -259,7 +301,7 @@ __device__ __forceinline__ void spiral_tile(int t, int& ox, int& oy) {
 // order, one tile per lane, the set bits of each tile taken lane-parallel.
 // Stops when k collected points lie closer than any pixel outside the
 // completed rings (every point that close has been collected).
-__device__ __forceinline__ void knn_rings(const BatchArgs& a, WaveLds* L, int x, int y, uint32_t k, int lane, Sel& s,
+__device__ __forceinline__ void knn_rings(const KnnView& a, WaveLds* L, int x, int y, uint32_t k, int lane, Sel& s,
                                           uint32_t& iters) {
     const int tx = x >> 3, ty = y >> 3;
     const int TX = (int)a.tiles_x, TY = (int)a.tiles_y;
@@ -269,21 +311,34 @@ __device__ __forceinline__ void knn_rings(const BatchArgs& a, WaveLds* L, int x,
     for (int t0 = 0; t0 < t_end; t0 += 4 * kWave) {
         ++iters;
         // 256 tiles per memory round trip, then their set bits lane-parallel
-        uint64_t word[4];
+        uint64_t word[4], nword[4];
+        uint32_t tmin[4];
         int gx[4], gy[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int t = t0 + u * kWave + lane;
-            word[u] = 0;
+            word[u] = nword[u] = 0;
+            tmin[u] = 0xffffffffu;
             gx[u] = gy[u] = 0;
             if (t < t_end) {
                 int ox, oy;
                 spiral_tile(t, ox, oy);
                 gx[u] = tx + ox;
                 gy[u] = ty + oy;
-                if (gx[u] >= 0 && gy[u] >= 0 && gx[u] < TX && gy[u] < TY)
-                    word[u] = a.mask[(uint32_t)gy[u] * a.tiles_x + (uint32_t)gx[u]];
+                if (gx[u] >= 0 && gy[u] >= 0 && gx[u] < TX && gy[u] < TY) {
+                    const uint32_t ti = (uint32_t)gy[u] * a.tiles_x + (uint32_t)gx[u];
+                    word[u] = a.mask[ti];
+                    if (a.mask_new) {
+                        nword[u] = a.mask_new[ti];
+                        tmin[u] = a.tile_min[ti];
+                    }
+                }
             }
+        }
+        if (a.mask_new) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (nword[u] && tmin[u] < a.n_at) word[u] |= filter_new(a, gx[u], gy[u], nword[u]);
         }
 #pragma unroll 1
         for (int u = 0; u < 4; ++u) {
@@ -320,7 +375,7 @@ __device__ __forceinline__ void knn_rings(const BatchArgs& a, WaveLds* L, int x,
 // Ring-ordered offset table scan: the first k resolved pixels in table
 // order are the k nearest (the table lists every offset within its radius
 // in canonical order). Returns the number found (<= k).
-__device__ __forceinline__ uint32_t knn_table(const BatchArgs& a, WaveLds* L, int x, int y, uint32_t k, int lane) {
+__device__ __forceinline__ uint32_t knn_table(const KnnView& a, WaveLds* L, int x, int y, uint32_t k, int lane) {
     uint32_t found = 0;
     for (uint32_t base = 0; base < a.offtab_n; base += kWave) {
         uint32_t t = base + lane;
@@ -329,7 +384,10 @@ __device__ __forceinline__ uint32_t knn_table(const BatchArgs& a, WaveLds* L, in
         if (t < a.offtab_n) {
             o = a.offtab[t];
             int X = x + packed_dx(o), Y = y + packed_dy(o);
-            hit = X >= 0 && Y >= 0 && X < (int)a.W && Y < (int)a.H && px_set(a.mask, a.tiles_x, X, Y);
+            const bool in = X >= 0 && Y >= 0 && X < (int)a.W && Y < (int)a.H;
+            hit = in && px_set(a.mask, a.tiles_x, X, Y);
+            if (in && !hit && a.mask_new && px_set(a.mask_new, a.tiles_x, X, Y))
+                hit = a.rank[(uint32_t)Y * a.W + (uint32_t)X] < a.n_at;
         }
         uint64_t m = __ballot(hit);
         uint32_t pre = (uint32_t)__popcll(m & lanemask_lt(lane));
@@ -338,6 +396,36 @@ __device__ __forceinline__ uint32_t knn_table(const BatchArgs& a, WaveLds* L, in
         if (found >= k) break;
     }
     return found < k ? found : k;
+}
+
+// k nearest of (x, y) among the view's resolved pixels (n_res of them), in
+// canonical order, into L->nd; returns the count (<= k).
+__device__ __forceinline__ uint32_t knn_search(const KnnView& v, WaveLds* L, int x, int y, uint32_t k, uint32_t n_res,
+                                               int lane, uint32_t& iters, uint32_t& buffered) {
+    uint32_t found = 0;
+    bool use_sel = n_res <= kBruteMax;
+    // the offset table pays off when about k / density <= 256 offsets reach
+    // k resolved pixels; sparser masks use the tile ring search
+    const bool sparse = (uint64_t)n_res * 256ull < (uint64_t)k * v.W * v.H;
+    if (!use_sel && !sparse) {
+        found = knn_table(v, L, x, y, k, lane);
+        use_sel = found < k && found < n_res;
+    } else if (!use_sel) {
+        use_sel = true;
+    }
+    iters = buffered = 0;
+    if (use_sel) {
+        Sel s{0u, ~0ull};
+        if (n_res <= kBruteMax) knn_list(v, L, n_res, x, y, k, lane, s);
+        else knn_rings(v, L, x, y, k, lane, s, iters);
+        buffered = s.count;
+        wave_sync();
+        sel_compact(L->keys, s, k, lane);
+        found = s.count;
+        if (lane < (int)found) L->nd[lane] = key_to_packed(L->keys[lane]);
+    }
+    wave_sync();
+    return found;
 }
 
 __device__ __forceinline__ bool coord_ok(const MapDesc& md, int x, int y) {
@@ -524,32 +612,16 @@ __device__ __forceinline__ uint32_t item_knn(const BatchArgs& a, uint32_t item, 
     if (a.ext_nbrs) {
         kf = a.ext_nbr_count[rel];
         if (lane < (int)kf) L->nd[lane] = a.ext_nbrs[(size_t)rel * k + lane];
+    } else if (a.pre_nbrs && item >= a.pre_base && item < a.pre_end) {
+        kf = a.pre_cnt[item - a.pre_base];
+        if (lane < (int)kf) L->nd[lane] = a.pre_nbrs[(size_t)(item - a.pre_base) * k + lane];
     } else {
-        uint32_t found = 0;
-        bool use_sel = n_res <= kBruteMax;
-        // the offset table pays off when about k / density <= 256 offsets reach
-        // k resolved pixels; sparser masks use the tile ring search
-        const bool sparse = (uint64_t)n_res * 256ull < (uint64_t)k * a.W * a.H;
-        if (!use_sel && !sparse) {
-            found = knn_table(a, L, x, y, k, lane);
-            use_sel = found < k && found < n_res;
-        } else if (!use_sel) {
-            use_sel = true;
-        }
-        if (use_sel) {
-            Sel s{0u, ~0ull};
-            uint32_t iters = 0;
-            if (n_res <= kBruteMax) knn_list(a, L, n_res, x, y, k, lane, s);
-            else knn_rings(a, L, x, y, k, lane, s, iters);
-            phase_clock(a, item, 7, lane);
-            phase_value(a, item, 8, iters, lane);
-            phase_value(a, item, 9, s.count, lane);
-            wave_sync();
-            sel_compact(L->keys, s, k, lane);
-            found = s.count;
-            if (lane < (int)found) L->nd[lane] = key_to_packed(L->keys[lane]);
-        }
-        kf = found;
+        const KnnView v{a.W, a.H, a.tiles_x, a.tiles_y, a.mask, a.rlist, a.offtab, a.offtab_n, nullptr, nullptr, nullptr, 0u};
+        uint32_t iters, buffered;
+        kf = knn_search(v, L, x, y, k, n_res, lane, iters, buffered);
+        phase_clock(a, item, 7, lane);
+        phase_value(a, item, 8, iters, lane);
+        phase_value(a, item, 9, buffered, lane);
     }
     wave_sync();
     if (a.dbg_nbrs) {
@@ -1050,11 +1122,61 @@ __global__ void k_commit_multi(const BatchArgs* units, const uint32_t* item_star
                a.state_w, a.mask_w, a.score_w);
 }
 
+// Neighbour lists of items [item0, item1) ahead of their batches, one
+// wavefront per item (the resolve kernels read them instead of searching).
+__global__ __launch_bounds__(kWave* kWavesPerWG) void k_knn_prefetch(PrefetchArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    WaveLds* L = reinterpret_cast<WaveLds*>(smem) + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    const uint32_t item = a.item0 + blockIdx.x * kWavesPerWG + threadIdx.x / kWave;
+    if (item >= a.item1) return;
+    const uint32_t rel = item - a.item0;
+    const uint32_t flat = a.work[item];
+    const uint32_t n_at = a.n_at[rel];
+    const KnnView v{a.W,        a.H,      a.tiles_x, a.tiles_y, a.mask, a.rlist, a.offtab, a.offtab_n,
+                    a.mask_new, a.tile_min, a.rank, n_at};
+    uint32_t iters, buffered;
+    const uint32_t kf = knn_search(v, L, (int)(flat % a.W), (int)(flat / a.W), a.k, n_at, lane, iters, buffered);
+    if (lane < (int)kf) a.out_nbrs[(size_t)rel * a.k + lane] = L->nd[lane];
+    if (lane == 0) a.out_cnt[rel] = kf;
+}
+
+__global__ void k_unpack(UnpackArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.n) {
+        const uint4 st = a.state[i];
+        if (a.rgba) a.rgba[i] = st.x;
+        if (a.coord) {
+            a.coord[3 * i] = st.y & 0xffffu;
+            a.coord[3 * i + 1] = st.y >> 16;
+            a.coord[3 * i + 2] = st.z & 0xffffu;
+        }
+        if (a.ids) {
+            a.ids[2 * i] = st.w;
+            a.ids[2 * i + 1] = st.z >> 16;
+        }
+    }
+    if (a.rscore && i < a.n_res) a.rscore[i] = a.score[a.rlist[i]];
+}
+
+__global__ void k_rank(const uint32_t* rlist, uint32_t lo, uint32_t hi, uint32_t* rank) {
+    const uint32_t q = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < hi) rank[rlist[q]] = q;
+}
+
+__global__ void k_mask_or(const uint32_t* rlist, uint32_t lo, uint32_t hi, uint32_t W, uint32_t tiles_x,
+                          uint64_t* mask, uint32_t* tile_min) {
+    const uint32_t q = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= hi) return;
+    const uint32_t p = rlist[q], px = p % W, py = p / W;
+    const uint32_t t = (py >> 3) * tiles_x + (px >> 3);
+    atomicOr((unsigned long long*)&mask[t], 1ull << (((py & 7u) << 3) | (px & 7u)));
+    if (tile_min) atomicMin(&tile_min[t], q);
+}
+
 // Random candidates of items [item0, item1): Pcg32::seed_from_u64(stage
 // seed + i + 1), m draws of (map, then x and y until the sample mask accepts).
-__global__ void k_rand(RandArgs a) {
-    uint32_t item = a.item0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (item >= a.item1) return;
+__device__ __forceinline__ void rand_item(const RandArgs& a, uint32_t item) {
     Pcg32 g = Pcg32::seed_from_u64(a.stage_seed + (uint64_t)item + 1u);
     uint2* out = a.out + (size_t)(item - a.item0) * a.m;
     for (uint32_t r = 0; r < a.m; ++r) {
@@ -1072,6 +1194,21 @@ __global__ void k_rand(RandArgs a) {
         }
         out[r] = make_uint2(x | (y << 16), map);
     }
+}
+
+__global__ void k_rand(RandArgs a) {
+    const uint32_t item = a.item0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (item < a.item1) rand_item(a, item);
+}
+
+// Random candidates of several jobs in one launch: thread g serves item
+// g - item_start[u] of unit u.
+__global__ void k_rand_multi(const RandArgs* units, const uint32_t* item_start, uint32_t n_units, uint32_t total) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const uint32_t u = find_unit(item_start, n_units, g);
+    const RandArgs& a = units[u];
+    rand_item(a, a.item0 + (g - item_start[u]));
 }
 
 __global__ void k_commit(CommitArgs a) {
@@ -1183,6 +1320,36 @@ hipError_t configure_kernels() {
                                (int)resolve_small_lds_bytes(false));
 }
 
+hipError_t launch_knn_prefetch(const PrefetchArgs& a, hipStream_t s) {
+    const uint32_t n = a.item1 - a.item0;
+    if (n == 0) return hipSuccess;
+    if (a.k == 0 || a.k > (uint32_t)kMaxK) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_knn_prefetch, dim3((n + kWavesPerWG - 1) / kWavesPerWG), dim3(kWave * kWavesPerWG),
+                       kWavesPerWG * sizeof(WaveLds), s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack(const UnpackArgs& a, hipStream_t s) {
+    const uint32_t n = a.n > a.n_res ? a.n : a.n_res;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_unpack, dim3((n + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_rank(const uint32_t* rlist, uint32_t lo, uint32_t hi, uint32_t* rank, hipStream_t s) {
+    if (hi <= lo) return hipSuccess;
+    hipLaunchKernelGGL(k_rank, dim3((hi - lo + 255) / 256), dim3(256), 0, s, rlist, lo, hi, rank);
+    return hipGetLastError();
+}
+
+hipError_t launch_mask_or(const uint32_t* rlist, uint32_t lo, uint32_t hi, uint32_t W, uint32_t tiles_x,
+                          uint64_t* mask, uint32_t* tile_min, hipStream_t s) {
+    if (hi <= lo) return hipSuccess;
+    hipLaunchKernelGGL(k_mask_or, dim3((hi - lo + 255) / 256), dim3(256), 0, s, rlist, lo, hi, W, tiles_x, mask,
+                       tile_min);
+    return hipGetLastError();
+}
+
 hipError_t launch_stats(const StatsArgs& a, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     uint32_t blocks = (a.n + 255) / 256;
@@ -1203,12 +1370,12 @@ uint32_t small_batch_max() { return kSmallWaves; }
 uint32_t items_per_wg(uint32_t team) { return kWavesPerWG / (team > 1 ? team : 1); }
 
 // Waves per item for a batch of n items: while the batch leaves most of the
-// device's wave slots idle (about 4096 resident waves of the resolve
-// kernel), each item gets a team of up to a workgroup.
-uint32_t team_for(uint64_t n) {
-    uint32_t T = kWavesPerWG;
-    while (T > 1 && (uint64_t)T * n > 4096ull) T >>= 1;
-    return T;
+// device's `waves` resident wave slots idle, each item gets a team of up to
+// team_max waves (at most a workgroup).
+uint32_t team_for(uint64_t n, uint32_t team_max, uint32_t waves) {
+    uint32_t T = team_max < (uint32_t)kWavesPerWG ? team_max : (uint32_t)kWavesPerWG;
+    while (T > 1 && (uint64_t)T * n > (uint64_t)waves) T >>= 1;
+    return T > 0 ? T : 1;
 }
 
 hipError_t launch_resolve_multi(const BatchArgs* units, const uint32_t* wg_start, uint32_t n_units, uint32_t n_wg,
@@ -1258,6 +1425,13 @@ hipError_t launch_resolve(const BatchArgs& a, bool guided, hipStream_t s) {
     } else {
         hipLaunchKernelGGL(k_resolve<false>, grid, block, lds, s, a);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_rand_multi(const RandArgs* units, const uint32_t* item_start, uint32_t n_units, uint32_t total,
+                             hipStream_t s) {
+    if (n_units == 0 || total == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rand_multi, dim3((total + 255) / 256), dim3(256), 0, s, units, item_start, n_units, total);
     return hipGetLastError();
 }
 

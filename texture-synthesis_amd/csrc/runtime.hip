@@ -17,6 +17,8 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <condition_variable>
+#include <deque>
 #include <thread>
 #include <vector>
 
@@ -235,9 +237,126 @@ void stage_luts(float cauchy, bool guided, float alpha_s, float* out512) {
 }  // namespace
 
 // ------------------------------------------------------------------------
+// Host-to-device uploads of the run loop go through pinned chunks: a copy
+// from pageable memory would make the host wait for the stream to drain.
+// A chunk is reused once the event recorded after its last copy completes.
+struct Staging {
+    static constexpr size_t kChunk = 4u << 20;
+    static constexpr int kChunks = 4;
+    char* buf[kChunks] = {};
+    hipEvent_t ev[kChunks] = {};
+    bool pending[kChunks] = {};
+    int cur = 0;
+    size_t used = 0;
+    hipError_t init() {
+        for (int q = 0; q < kChunks; ++q) {
+            hipError_t e = hipHostMalloc((void**)&buf[q], kChunk);
+            if (e != hipSuccess) return e;
+            if ((e = hipEventCreateWithFlags(&ev[q], hipEventDisableTiming)) != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+    ~Staging() {
+        for (int q = 0; q < kChunks; ++q) {
+            if (buf[q]) (void)hipHostFree(buf[q]);
+            if (ev[q]) (void)hipEventDestroy(ev[q]);
+        }
+    }
+    hipError_t upload(void* dst, const void* src, size_t n, hipStream_t s) {
+        const char* from = static_cast<const char*>(src);
+        char* to = static_cast<char*>(dst);
+        while (n > 0) {
+            if (used >= kChunk) {
+                hipError_t e = next(s);
+                if (e != hipSuccess) return e;
+            }
+            const size_t piece = std::min(n, kChunk - used);
+            memcpy(buf[cur] + used, from, piece);
+            hipError_t e = hipMemcpyAsync(to, buf[cur] + used, piece, hipMemcpyHostToDevice, s);
+            if (e != hipSuccess) return e;
+            used += (piece + 255) & ~(size_t)255;
+            from += piece;
+            to += piece;
+            n -= piece;
+        }
+        return hipSuccess;
+    }
+    hipError_t next(hipStream_t s) {
+        hipError_t e = hipEventRecord(ev[cur], s);
+        if (e != hipSuccess) return e;
+        pending[cur] = true;
+        cur = (cur + 1) % kChunks;
+        used = 0;
+        if (pending[cur]) {
+            if ((e = hipEventSynchronize(ev[cur])) != hipSuccess) return e;
+            pending[cur] = false;
+        }
+        return hipSuccess;
+    }
+};
+
+struct tsg_job;
+namespace {
+void picks_work(tsg_job* j);
+}  // namespace
+
+// Host threads that compute jobs' pick orders (the reference's swap_remove
+// chain over the unresolved list) while the device works on earlier stages.
+struct PicksPool {
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<tsg_job*> queue;
+    std::vector<std::thread> workers;
+    bool stop = false;
+    void submit(tsg_job* j) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            queue.push_back(j);
+            if (workers.size() < std::min(8u, std::max(1u, std::thread::hardware_concurrency() / 2)) &&
+                workers.size() < queue.size() + 1)
+                workers.emplace_back([this] { loop(); });
+        }
+        cv.notify_one();
+    }
+    // true when j was still queued (and is now removed)
+    bool remove(tsg_job* j) {
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto it = queue.begin(); it != queue.end(); ++it)
+            if (*it == j) {
+                queue.erase(it);
+                return true;
+            }
+        return false;
+    }
+    void loop() {
+        for (;;) {
+            tsg_job* j;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return stop || !queue.empty(); });
+                if (stop) return;
+                j = queue.front();
+                queue.pop_front();
+            }
+            picks_work(j);
+        }
+    }
+    ~PicksPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        cv.notify_all();
+        for (auto& t : workers) t.join();
+    }
+};
+
 struct tsg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;  // work that overlaps the stage loop (random candidates ahead)
+    Staging staging, staging_side;  // one per stream: a chunk's reuse waits on its stream's event
+    PicksPool picks;
     DBuf<int32_t> offtab;
     uint32_t offtab_n = 0;
     DBuf<float> dummy_lut;
@@ -248,6 +367,7 @@ struct tsg_job {
     tsg_ctx* ctx = nullptr;
     tsg_params prm{};
     tsg_schedule sched{};
+    tsg_tuning tune{};
     uint32_t W = 0, H = 0, tiles_x = 0, tiles_y = 0;
     size_t N = 0;
     uint32_t n_ex = 0, levels = 1;
@@ -272,6 +392,29 @@ struct tsg_job {
     // scratch
     DBuf<uint4> result;
     DBuf<uint2> rc;
+    // random candidates of whole stages, generated a stage ahead on the side
+    // stream (single-job runs): buffer t % 2 holds stage t, in pieces
+    // published by events; done[b] marks the end of the stage that read b
+    // Single-job runs compute everything content-independent a stage ahead
+    // on the side stream, into buffer set t % 2: stage t's random
+    // candidates, its picks in the resolved list and their ranks, the mask
+    // at its end, and every item's neighbour list (the resolved set at any
+    // moment follows from the pick order alone). `ready` publishes the set;
+    // `done` marks the end of the stage that read it.
+    struct Ahead {
+        DBuf<uint2> rc;
+        DBuf<int32_t> nbrs;
+        DBuf<uint32_t> cnt, n_at;
+        DBuf<uint64_t> mend, mnew;  // resolved mask at the stage end; the stage's picks
+        DBuf<uint32_t> tmin;        // per tile: least position of its picks
+        hipEvent_t ready = nullptr, done = nullptr;
+        bool done_recorded = false;
+        int stage = -1;
+        uint64_t n_open = 0, S_eff = 0, w_end = 0;
+    } ahead[2];
+    bool ahead_on = false;
+    uint64_t ahead_open = 0;  // resolved count at the open of the next stage to prepare
+    int ahead_next = 0;       // next stage to prepare
     DBuf<float> item_score;
     DBuf<float> lut;
     DBuf<uint32_t> overflow;
@@ -279,8 +422,14 @@ struct tsg_job {
     DBuf<unsigned long long> st_cnt;
     DBuf<uint32_t> done_counter;
     DBuf<uint32_t> small_tab;  // batch bounds and resolved counts of small runs
+    // neighbour lists computed ahead of latency-bound batches
+    DBuf<uint32_t> rank;       // per pixel: position in the resolved list (UINT32_MAX: none)
+    DBuf<uint64_t> mend;       // pixels resolved inside a prefetch window
+    DBuf<uint32_t> tmin;       // per tile: least resolved-list position among them
+    DBuf<uint32_t> pre_n;      // per window item: resolved count at its batch start
+    DBuf<int32_t> pre_nbrs;    // per window item: k packed offsets
+    DBuf<uint32_t> pre_cnt;
     size_t small_used = 0;
-    std::vector<std::vector<uint32_t>> small_keep;
     // host bookkeeping
     std::vector<uint32_t> unresolved;
     size_t n_res = 0, locked = 0, N0 = 0;
@@ -290,7 +439,15 @@ struct tsg_job {
     bool ran = false;
     bool state_ready = false;
     size_t rc_budget = 32u << 20;  // bytes of random candidates generated per chunk
+    // pick orders computed by the context's pool: picks_ready[t] and
+    // picks_keep[t] are published under pk_mu
     std::vector<uint8_t> picks_ready;
+    std::mutex pk_mu;
+    std::condition_variable pk_cv;
+    int pk_state = 0;  // 0 not submitted, 1 queued, 2 computing, 3 finished
+    bool pk_cancel = false;
+    unsigned pk_threads = 4;  // threads of one pick chain's index draws (the job thread keeps a core)
+    uint64_t pk_start_res = 0;
     uint64_t overall_total = 0, overall_cur = 0;
     hipEvent_t ev_run0 = nullptr, ev_run1 = nullptr;
     // cursor of the resumable stage driver
@@ -300,7 +457,6 @@ struct tsg_job {
         uint32_t level = 0;
         uint64_t pss = 0, S = 0, S_eff = 0, redo = 0, redo_eff = 0, i = 0;
         uint64_t rc_lo = 0, rc_hi = 0;
-        uint64_t planned_res = 0;
         BatchArgs a{};
     } step;
     double run_ms = 0.0;
@@ -317,11 +473,21 @@ struct tsg_job {
     std::vector<Timed> timed;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_next = 0;
-    double kern_ms[5] = {0, 0, 0, 0, 0};
-    uint64_t kern_items[5] = {0, 0, 0, 0, 0};
-    uint64_t kern_launches[5] = {0, 0, 0, 0, 0};
+    double kern_ms[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t kern_items[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t kern_launches[6] = {0, 0, 0, 0, 0, 0};
     ~tsg_job() {
+        // the pick worker must be done with this job
+        if (ctx && !ctx->picks.remove(this)) {
+            std::unique_lock<std::mutex> lk(pk_mu);
+            pk_cancel = true;
+            pk_cv.wait(lk, [&] { return pk_state != 1 && pk_state != 2; });
+        }
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+        for (Ahead& a : ahead) {
+            if (a.ready) (void)hipEventDestroy(a.ready);
+            if (a.done) (void)hipEventDestroy(a.done);
+        }
         if (ev_run0) (void)hipEventDestroy(ev_run0);
         if (ev_run1) (void)hipEventDestroy(ev_run1);
     }
@@ -343,7 +509,7 @@ std::vector<uint32_t> pick_new(tsg_job* j, uint64_t stage_seed, uint64_t i0, uin
     size_t L0 = j->unresolved.size();
     n = std::min<uint64_t>(n, L0);
     std::vector<uint64_t> idx(n);
-    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    unsigned nt = std::max(1u, std::min(j->pk_threads, std::thread::hardware_concurrency()));
     if (n < 4096) nt = 1;
     std::vector<std::thread> th;
     for (unsigned t = 0; t < nt; ++t) {
@@ -441,28 +607,35 @@ int init_state(tsg_job* j) {
     HIP_TRY(hipMemcpyAsync(j->mask.p, mask.data(), mask.size() * 8, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemsetAsync(j->score.p, 0, N * sizeof(float), s));
     if (!rl.empty()) HIP_TRY(hipMemcpyAsync(j->rlist.p, rl.data(), rl.size() * 4, hipMemcpyHostToDevice, s));
+    std::vector<uint32_t> rank(N, UINT32_MAX);
+    for (size_t q = 0; q < rl.size(); ++q) rank[rl[q]] = (uint32_t)q;
+    HIP_TRY(hipMemcpyAsync(j->rank.p, rank.data(), N * 4, hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
     return TSG_OK;
 }
 
-uint64_t batch_end(const tsg_job* j, uint64_t i, uint64_t redo, uint64_t S_eff) {
+// End of the batch starting at item i when n_res pixels are resolved.
+uint64_t batch_end_at(const tsg_job* j, uint64_t i, uint64_t n_res, uint64_t redo, uint64_t S_eff) {
     uint64_t redo_eff = std::min<uint64_t>(redo, S_eff);
     uint64_t seg_end = i < redo_eff ? redo_eff : S_eff;
     uint64_t b;
     if (j->sched.px_per_batch > 0) {
         b = j->sched.px_per_batch;
     } else {
-        uint64_t base = j->n_res > j->locked ? j->n_res - j->locked : 0;
+        uint64_t base = n_res > j->locked ? n_res - j->locked : 0;
         b = base >> (i < redo_eff ? j->sched.redo_shift : j->sched.new_shift);
         if (b < 1) b = 1;
         if (j->sched.max_batch > 0 && b > j->sched.max_batch) b = j->sched.max_batch;
     }
     return std::min(i + b, seg_end);
 }
+uint64_t batch_end(const tsg_job* j, uint64_t i, uint64_t redo, uint64_t S_eff) {
+    return batch_end_at(j, i, j->n_res, redo, S_eff);
+}
 
-enum { K_RESOLVE = 0, K_COMMIT = 1, K_RAND = 2, K_RECOLOR = 3, K_STATS = 4 };
-constexpr uint64_t kFuseMax = 2048;  // batches up to this size commit inside the resolve launch
+enum { K_RESOLVE = 0, K_COMMIT = 1, K_RAND = 2, K_RECOLOR = 3, K_STATS = 4, K_PREFETCH = 5 };
 constexpr size_t kEngineRcBudget = 4u << 20;  // random-candidate chunk of a job under the engine
+constexpr uint64_t kPrefetchCap = 65536;       // items per prefetched kNN window
 
 hipError_t next_event(tsg_job* j, hipEvent_t* e) {
     if (j->ev_next == j->ev_pool.size()) {
@@ -475,36 +648,176 @@ hipError_t next_event(tsg_job* j, hipEvent_t* e) {
     return hipSuccess;
 }
 
-// Runs `launch` on the job stream, bracketed by events when profiling.
+// Runs `launch` (on stream s), bracketed by events when profiling.
 template <class F>
-hipError_t timed_launch(tsg_job* j, int kind, uint64_t items, F&& launch) {
+hipError_t timed_launch_on(tsg_job* j, hipStream_t s, int kind, uint64_t items, F&& launch) {
     if (!j->profile) return launch();
     tsg_job::Timed t{kind, items, nullptr, nullptr};
     hipError_t e;
     if ((e = next_event(j, &t.a)) != hipSuccess) return e;
     if ((e = next_event(j, &t.b)) != hipSuccess) return e;
-    if ((e = hipEventRecord(t.a, j->ctx->stream)) != hipSuccess) return e;
+    if ((e = hipEventRecord(t.a, s)) != hipSuccess) return e;
     if ((e = launch()) != hipSuccess) return e;
-    if ((e = hipEventRecord(t.b, j->ctx->stream)) != hipSuccess) return e;
+    if ((e = hipEventRecord(t.b, s)) != hipSuccess) return e;
     j->timed.push_back(t);
     return hipSuccess;
 }
+template <class F>
+hipError_t timed_launch(tsg_job* j, int kind, uint64_t items, F&& launch) {
+    return timed_launch_on(j, j->ctx->stream, kind, items, launch);
+}
+
+constexpr uint64_t kAheadBudget = 16ull << 30;  // device bytes of the stage-ahead buffers
 
 // ---- resumable stage driver (Generator::resolve, ms.rs:786-1050) ----
 
-// Pick order of stage t's new pixels (computed once; the chain depends only
-// on counts and seeds, not on image content).
-void stage_picks(tsg_job* j, int t) {
-    tsg_job::Step& st = j->step;
-    if (j->picks_ready[t]) return;
+// Pick orders of every stage's new pixels, in stage order (the chain depends
+// only on counts and seeds, not on image content). Runs on a pool thread;
+// each stage is published as soon as it is done.
+void picks_work(tsg_job* j) {
+    {
+        std::lock_guard<std::mutex> lk(j->pk_mu);
+        j->pk_state = 2;
+    }
     const int P = (int)j->prm.backtrack_stages;
+    uint64_t planned = j->pk_start_res;
+    for (int t = 0; t <= P; ++t) {
+        {
+            std::lock_guard<std::mutex> lk(j->pk_mu);
+            if (j->pk_cancel) break;
+            if (j->picks_ready[t]) {
+                planned += j->picks_keep[t].size();
+                continue;
+            }
+        }
+        const int ps = P - t;
+        const uint64_t pss = (uint64_t)Pcg32::seed_from_u64(j->prm.seed + (uint64_t)ps).next_u32();
+        const uint64_t S = stage_size(j, ps);
+        const uint64_t redo = planned - j->locked;
+        std::vector<uint32_t> picks;
+        if (S > redo) picks = pick_new(j, pss, redo, S - redo);
+        planned += picks.size();
+        {
+            std::lock_guard<std::mutex> lk(j->pk_mu);
+            j->picks_keep[t] = std::move(picks);
+            j->picks_ready[t] = 1;
+        }
+        j->pk_cv.notify_all();
+    }
+    {
+        std::lock_guard<std::mutex> lk(j->pk_mu);
+        j->pk_state = 3;
+    }
+    j->pk_cv.notify_all();
+}
+
+bool picks_ready_now(tsg_job* j, int t) {
+    std::lock_guard<std::mutex> lk(j->pk_mu);
+    return j->picks_ready[t] != 0;
+}
+
+int wait_picks(tsg_job* j, int t) {
+    std::unique_lock<std::mutex> lk(j->pk_mu);
+    j->pk_cv.wait(lk, [&] { return j->picks_ready[t] || j->pk_state == 3; });
+    if (!j->picks_ready[t]) return fail(TSG_ERR_DEVICE, "pick order computation stopped");
+    return TSG_OK;
+}
+
+// Device bytes of the stage-ahead buffers (two sets sized for the largest stage).
+uint64_t ahead_bytes(const tsg_job* j) {
+    const uint64_t S = std::max<uint64_t>(1, stage_size(j, 0));
+    const uint64_t per = S * (j->prm.random_sample_locations * sizeof(uint2) + j->prm.nearest_neighbors * 4 + 8) +
+                         (uint64_t)j->tiles_x * j->tiles_y * 20;
+    return 2 * per;
+}
+
+// Prepares stage ahead_next on the side stream (see tsg_job::Ahead). The
+// counts follow the stage driver: a stage opens with `ahead_open` resolved
+// pixels, redoes those past `locked`, and appends its picks.
+int ahead_stage(tsg_job* j) {
+    const int P = (int)j->prm.backtrack_stages;
+    const int t = j->ahead_next;
+    if (t > P) return TSG_OK;
+    int rc = wait_picks(j, t);
+    if (rc) return rc;
+    tsg_job::Ahead& A = j->ahead[t % 2];
+    hipStream_t side = j->ctx->side;
+    const uint32_t nf = (uint32_t)j->filt.size();
+    const uint32_t m = (uint32_t)j->prm.random_sample_locations;
+    const uint32_t k = j->prm.nearest_neighbors;
     const int ps = P - t;
-    const uint64_t pss = (uint64_t)Pcg32::seed_from_u64(j->prm.seed + (uint64_t)ps).next_u32();
+    const uint32_t level = t == 0 ? 0u : (uint32_t)std::min(t, P - 1);
     const uint64_t S = stage_size(j, ps);
-    const uint64_t redo = st.planned_res - j->locked;
-    if (S > redo) j->picks_keep[t] = pick_new(j, pss, redo, S - redo);
-    st.planned_res += j->picks_keep[t].size();
-    j->picks_ready[t] = 1;
+    const uint64_t pss = (uint64_t)Pcg32::seed_from_u64(j->prm.seed + (uint64_t)ps).next_u32();
+    const uint64_t n_open = j->ahead_open;
+    const uint64_t redo = n_open - j->locked;
+    const std::vector<uint32_t>& picks = j->picks_keep[t];
+    const uint64_t S_eff = S > redo ? redo + picks.size() : S;
+    const uint64_t redo_eff = std::min(redo, S_eff);
+    if (A.done_recorded) HIP_TRY(hipStreamWaitEvent(side, A.done, 0));
+    // random candidates of every item (ms.rs:549-599: stage seed, item index, level maps)
+    if (S_eff) {
+        RandArgs ra{j->maps.p + (size_t)level * nf, nf, pss, 0u, (uint32_t)S_eff, m, A.rc.p, j->overflow.p};
+        HIP_TRY(timed_launch_on(j, side, K_RAND, S_eff, [&] { return launch_rand(ra, side); }));
+    }
+    // picks into the resolved list, their ranks, the stage's picks as a mask
+    // (with each tile's least rank) and the mask at the stage end
+    const size_t tiles = (size_t)j->tiles_x * j->tiles_y;
+    const uint64_t* open_mask = t == 0 ? j->mask.p : j->ahead[(t + 1) % 2].mend.p;
+    HIP_TRY(hipMemcpyAsync(A.mend.p, open_mask, tiles * 8, hipMemcpyDeviceToDevice, side));
+    HIP_TRY(hipMemsetAsync(A.mnew.p, 0, tiles * 8, side));
+    HIP_TRY(hipMemsetAsync(A.tmin.p, 0xff, tiles * 4, side));
+    if (!picks.empty()) {
+        const uint32_t lo = (uint32_t)n_open, hi = (uint32_t)(n_open + picks.size());
+        HIP_TRY(j->ctx->staging_side.upload(j->rlist.p + n_open, picks.data(), picks.size() * 4, side));
+        HIP_TRY(launch_rank(j->rlist.p, lo, hi, j->rank.p, side));
+        HIP_TRY(launch_mask_or(j->rlist.p, lo, hi, j->W, j->tiles_x, A.mnew.p, A.tmin.p, side));
+        HIP_TRY(launch_mask_or(j->rlist.p, lo, hi, j->W, j->tiles_x, A.mend.p, nullptr, side));
+    }
+    // resolved count at each item's batch start; neighbour lists of the
+    // items in the leading latency-bound batches (the rest search in-kernel)
+    std::vector<uint32_t> n_at(S_eff);
+    uint64_t i = 0, nres = n_open, w_end = 0;
+    bool window = true;
+    while (i < S_eff) {
+        const uint64_t e = batch_end_at(j, i, nres, redo, S_eff);
+        std::fill(n_at.begin() + i, n_at.begin() + e, (uint32_t)nres);
+        if (window && e - i <= j->tune.prefetch_max) w_end = e;
+        else window = false;
+        if (e > redo_eff) nres += e - std::max(i, redo_eff);
+        i = e;
+    }
+    if (w_end) {
+        HIP_TRY(j->ctx->staging_side.upload(A.n_at.p, n_at.data(), w_end * 4, side));
+        PrefetchArgs pa{};
+        pa.W = j->W;
+        pa.H = j->H;
+        pa.tiles_x = j->tiles_x;
+        pa.tiles_y = j->tiles_y;
+        pa.mask = open_mask;
+        pa.mask_new = A.mnew.p;
+        pa.tile_min = A.tmin.p;
+        pa.rank = j->rank.p;
+        pa.rlist = j->rlist.p;
+        pa.work = j->rlist.p + j->locked;
+        pa.n_at = A.n_at.p;
+        pa.offtab = j->ctx->offtab.p;
+        pa.offtab_n = j->ctx->offtab_n;
+        pa.k = k;
+        pa.item0 = 0;
+        pa.item1 = (uint32_t)w_end;
+        pa.out_nbrs = A.nbrs.p;
+        pa.out_cnt = A.cnt.p;
+        HIP_TRY(timed_launch_on(j, side, K_PREFETCH, w_end, [&] { return launch_knn_prefetch(pa, side); }));
+    }
+    HIP_TRY(hipEventRecord(A.ready, side));
+    A.stage = t;
+    A.n_open = n_open;
+    A.S_eff = S_eff;
+    A.w_end = w_end;
+    j->ahead_open = nres;
+    j->ahead_next = t + 1;
+    return TSG_OK;
 }
 
 int step_begin(tsg_job* j) {
@@ -513,19 +826,22 @@ int step_begin(tsg_job* j) {
     if (j->ran || j->step.active) return fail(TSG_ERR_INVALID_ARGUMENT, "a job runs once (Session::run consumes the session)");
     j->timed.clear();
     j->ev_next = 0;
-    for (int q = 0; q < 5; ++q) j->kern_ms[q] = 0.0, j->kern_items[q] = 0, j->kern_launches[q] = 0;
+    for (int q = 0; q < 6; ++q) j->kern_ms[q] = 0.0, j->kern_items[q] = 0, j->kern_launches[q] = 0;
     const int P = (int)j->prm.backtrack_stages;
     j->st_items.assign(P + 1, 0);
     HIP_TRY(hipMemsetAsync(j->st_sum.p, 0, (P + 1) * sizeof(double), s));
     HIP_TRY(hipMemsetAsync(j->st_cnt.p, 0, (P + 1) * sizeof(unsigned long long), s));
     HIP_TRY(hipMemsetAsync(j->overflow.p, 0, 4, s));
     HIP_TRY(hipMemsetAsync(j->done_counter.p, 0, 4, s));
-    if (j->picks_keep.size() != (size_t)P + 1) {
+    {
+        std::lock_guard<std::mutex> lk(j->pk_mu);
         j->picks_keep.assign(P + 1, {});
         j->picks_ready.assign(P + 1, 0);
+        j->pk_start_res = j->n_res;
+        j->pk_state = 1;
     }
+    j->ctx->picks.submit(j);
     j->lut_keep.assign(P + 1, std::vector<float>(512));
-    j->small_keep.clear();
     j->small_used = 0;
     j->n_batches = 0;
     // Table arena of the small-batch runs: a run of nb batches takes 2 nb + 1
@@ -551,7 +867,6 @@ int step_begin(tsg_job* j) {
     j->overall_cur = 0;
     j->step = tsg_job::Step{};
     j->step.active = true;
-    j->step.planned_res = j->n_res;
     HIP_TRY(hipEventCreate(&j->ev_run0));
     HIP_TRY(hipEventCreate(&j->ev_run1));
     HIP_TRY(hipEventRecord(j->ev_run0, s));
@@ -584,22 +899,32 @@ int step_open_stage(tsg_job* j) {
     stage_luts(j->prm.cauchy_dispersion, j->guided, alpha_s, j->lut_keep[t].data());
     uint32_t lut_nan = 0;
     for (float v : j->lut_keep[t]) lut_nan |= (uint32_t)std::isnan(v);
-    HIP_TRY(hipMemcpyAsync(j->lut.p, j->lut_keep[t].data(), 512 * 4, hipMemcpyHostToDevice, s));
-    if (!j->picks_ready[t]) {
-        // picks are computed in stage order; planned_res tracks the count
-        st.planned_res = j->n_res;
-        stage_picks(j, t);
-    }
+    HIP_TRY(j->ctx->staging.upload(j->lut.p, j->lut_keep[t].data(), 512 * 4, s));
+    int rc = wait_picks(j, t);
+    if (rc) return rc;
     st.S_eff = st.S;
     if (st.S > st.redo) {
         st.S_eff = st.redo + j->picks_keep[t].size();
-        if (!j->picks_keep[t].empty())
-            HIP_TRY(hipMemcpyAsync(j->rlist.p + j->locked + st.redo, j->picks_keep[t].data(),
-                                   j->picks_keep[t].size() * 4, hipMemcpyHostToDevice, s));
+        if (!j->picks_keep[t].empty() && !j->ahead_on) {
+            HIP_TRY(j->ctx->staging.upload(j->rlist.p + j->locked + st.redo, j->picks_keep[t].data(),
+                                           j->picks_keep[t].size() * 4, s));
+            const uint32_t lo = (uint32_t)(j->locked + st.redo);
+            HIP_TRY(launch_rank(j->rlist.p, lo, lo + (uint32_t)j->picks_keep[t].size(), j->rank.p, s));
+        }
     }
     st.redo_eff = std::min(st.redo, st.S_eff);
     st.i = 0;
     st.rc_lo = st.rc_hi = 0;
+    if (j->ahead_on) {
+        // this stage's set was prepared during the previous stage; the next
+        // stage's preparation overlaps this one
+        const tsg_job::Ahead& A = j->ahead[t % 2];
+        if (A.stage != t || A.n_open != j->n_res || A.S_eff != st.S_eff)
+            return fail(TSG_ERR_DEVICE, "stage-ahead buffers out of step");
+        HIP_TRY(hipStreamWaitEvent(s, j->ahead[t % 2].ready, 0));
+        int rc2 = ahead_stage(j);
+        if (rc2) return rc2;
+    }
     BatchArgs& a = st.a;
     a = BatchArgs{};
     a.W = j->W;
@@ -628,23 +953,46 @@ int step_open_stage(tsg_job* j) {
     a.mask_w = j->mask.p;
     a.score_w = j->score.p;
     a.result = j->result.p;
+    if (j->ahead_on) {
+        // every item of the stage has its candidates and neighbour list
+        a.rcands = j->ahead[t % 2].rc.p;
+        a.rc_base = 0;
+        st.rc_lo = 0;
+        st.rc_hi = st.S_eff;
+        a.pre_nbrs = j->ahead[t % 2].nbrs.p;
+        a.pre_cnt = j->ahead[t % 2].cnt.p;
+        a.pre_base = 0;
+        a.pre_end = (uint32_t)j->ahead[t % 2].w_end;
+    }
     st.open = true;
     return TSG_OK;
 }
 
-// Random candidates must cover items [lo, hi) of the open stage.
-int step_ensure_rc(tsg_job* j, uint64_t lo, uint64_t hi) {
+// Random candidates must cover items [lo, hi) of the open stage. When they
+// do not, plans a generation (into *ra) and points the batch arguments at
+// it; the caller launches it before the batch.
+int step_plan_rc(tsg_job* j, uint64_t lo, uint64_t hi, RandArgs* ra, bool* need) {
     tsg_job::Step& st = j->step;
+    *need = false;
     if (lo >= st.rc_lo && hi <= st.rc_hi) return TSG_OK;
     const uint32_t m = (uint32_t)j->prm.random_sample_locations;
     uint64_t cap_items = std::max<uint64_t>(hi - lo, (uint64_t)j->rc_budget / (8ull * std::max(1u, m)));
     st.rc_lo = lo;
     st.rc_hi = std::min<uint64_t>(st.S_eff, lo + cap_items);
     HIP_TRY(j->rc.reserve((st.rc_hi - st.rc_lo) * m));
-    RandArgs ra{st.a.maps, st.a.n_maps, st.pss, (uint32_t)st.rc_lo, (uint32_t)st.rc_hi, m, j->rc.p, j->overflow.p};
-    HIP_TRY(timed_launch(j, K_RAND, st.rc_hi - st.rc_lo, [&] { return launch_rand(ra, j->ctx->stream); }));
+    *ra = RandArgs{st.a.maps, st.a.n_maps, st.pss, (uint32_t)st.rc_lo, (uint32_t)st.rc_hi, m, j->rc.p, j->overflow.p};
     st.a.rcands = j->rc.p;
     st.a.rc_base = (uint32_t)st.rc_lo;
+    *need = true;
+    return TSG_OK;
+}
+
+int step_ensure_rc(tsg_job* j, uint64_t lo, uint64_t hi) {
+    RandArgs ra;
+    bool need;
+    int rc = step_plan_rc(j, lo, hi, &ra, &need);
+    if (rc || !need) return rc;
+    HIP_TRY(timed_launch(j, K_RAND, ra.item1 - ra.item0, [&] { return launch_rand(ra, j->ctx->stream); }));
     return TSG_OK;
 }
 
@@ -655,6 +1003,78 @@ bool step_peek(tsg_job* j, uint64_t& i0, uint64_t& i1) {
     i0 = st.i;
     i1 = batch_end(j, st.i, st.redo, st.S_eff);
     return true;
+}
+
+// Whether batch [i0, i1) is latency-bound and its neighbour lists are not
+// computed yet.
+bool step_wants_prefetch(const tsg_job* j, uint64_t i0, uint64_t i1) {
+    const BatchArgs& a = j->step.a;
+    if (j->ahead_on) return false;
+    if (i1 - i0 > j->tune.prefetch_max) return false;
+    return !(a.pre_nbrs && i0 >= a.pre_base && i1 <= a.pre_end);
+}
+
+// Neighbour lists of the window of consecutive latency-bound batches that
+// starts at i0 (each at most prefetch_max items, inside i0's segment, at most
+// kPrefetchCap items), in one launch: item i searches the resolved set of
+// its batch start, i.e. the pixels listed before position n_at(i) of the
+// resolved list. The mask holds the window's pixels too; rank filters them.
+int step_prefetch(tsg_job* j, uint64_t i0) {
+    hipStream_t s = j->ctx->stream;
+    tsg_job::Step& st = j->step;
+    std::vector<uint32_t> n_at;
+    const bool redo = i0 < st.redo_eff;
+    const uint64_t seg_end = redo ? st.redo_eff : st.S_eff;
+    uint64_t i = i0, nres = j->n_res;
+    while (i < seg_end) {
+        const uint64_t e = batch_end_at(j, i, nres, st.redo, st.S_eff);
+        if (e - i > j->tune.prefetch_max || e - i0 > kPrefetchCap) break;
+        n_at.insert(n_at.end(), e - i, (uint32_t)nres);
+        if (e > st.redo_eff) nres += e - std::max(i, st.redo_eff);
+        i = e;
+    }
+    const uint64_t w1 = i;
+    if (w1 <= i0) return TSG_OK;
+    const uint32_t k = j->prm.nearest_neighbors;
+    const uint64_t n = w1 - i0;
+    HIP_TRY(j->pre_n.reserve(std::min<uint64_t>(kPrefetchCap, j->N)));
+    HIP_TRY(j->pre_cnt.reserve(std::min<uint64_t>(kPrefetchCap, j->N)));
+    HIP_TRY(j->pre_nbrs.reserve(std::min<uint64_t>(kPrefetchCap, j->N) * k));
+    HIP_TRY(j->ctx->staging.upload(j->pre_n.p, n_at.data(), n * 4, s));
+    PrefetchArgs pa{};
+    pa.W = j->W;
+    pa.H = j->H;
+    pa.tiles_x = j->tiles_x;
+    pa.tiles_y = j->tiles_y;
+    pa.rlist = j->rlist.p;
+    pa.work = j->rlist.p + j->locked;
+    pa.n_at = j->pre_n.p;
+    pa.offtab = j->ctx->offtab.p;
+    pa.offtab_n = j->ctx->offtab_n;
+    pa.k = k;
+    pa.item0 = (uint32_t)i0;
+    pa.item1 = (uint32_t)w1;
+    pa.out_nbrs = j->pre_nbrs.p;
+    pa.out_cnt = j->pre_cnt.p;
+    // the live mask as the window opens (read before the window's commits),
+    // plus, in a new-pixel segment, the window's own pixels filtered by rank
+    pa.mask = j->mask.p;
+    if (!redo) {
+        const size_t tiles = (size_t)j->tiles_x * j->tiles_y;
+        HIP_TRY(hipMemsetAsync(j->mend.p, 0, tiles * 8, s));
+        HIP_TRY(hipMemsetAsync(j->tmin.p, 0xff, tiles * 4, s));
+        HIP_TRY(launch_mask_or(j->rlist.p, (uint32_t)(j->locked + i0), (uint32_t)(j->locked + w1), j->W, j->tiles_x,
+                               j->mend.p, j->tmin.p, s));
+        pa.mask_new = j->mend.p;
+        pa.tile_min = j->tmin.p;
+        pa.rank = j->rank.p;
+    }
+    HIP_TRY(timed_launch(j, K_PREFETCH, n, [&] { return launch_knn_prefetch(pa, s); }));
+    st.a.pre_nbrs = j->pre_nbrs.p;
+    st.a.pre_cnt = j->pre_cnt.p;
+    st.a.pre_base = (uint32_t)i0;
+    st.a.pre_end = (uint32_t)w1;
+    return TSG_OK;
 }
 
 // The batch [i0, i1) has been enqueued: advance the cursor and counts.
@@ -670,6 +1090,12 @@ int step_close_stage(tsg_job* j) {
     tsg_job::Step& st = j->step;
     StatsArgs sa{j->item_score.p, (uint32_t)st.S_eff, j->st_sum.p + st.t, j->st_cnt.p + st.t};
     HIP_TRY(timed_launch(j, K_STATS, st.S_eff, [&] { return launch_stats(sa, j->ctx->stream); }));
+    if (j->ahead_on) {
+        tsg_job::Ahead& A = j->ahead[st.t % 2];
+        if (!A.done) HIP_TRY(hipEventCreateWithFlags(&A.done, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(A.done, j->ctx->stream));
+        A.done_recorded = true;
+    }
     j->st_items[st.t] = st.S_eff;
     j->overall_cur += st.S;
     st.open = false;
@@ -680,7 +1106,14 @@ int step_close_stage(tsg_job* j) {
 bool step_done(const tsg_job* j) { return j->step.t > (int)j->prm.backtrack_stages; }
 
 // After the job's last launch: record the end event; step_finish waits.
+// Side-stream work (pieces past the last item used) is joined first.
 int step_end(tsg_job* j) {
+    if (j->ahead_on) {
+        hipEvent_t e;
+        HIP_TRY(next_event(j, &e));
+        HIP_TRY(hipEventRecord(e, j->ctx->side));
+        HIP_TRY(hipStreamWaitEvent(j->ctx->stream, e, 0));
+    }
     HIP_TRY(hipEventRecord(j->ev_run1, j->ctx->stream));
     return TSG_OK;
 }
@@ -710,11 +1143,35 @@ int step_finish(tsg_job* j) {
 
 // Single job: runs of small batches go to one single-workgroup launch; other
 // batches get one k_resolve launch, committing in its last workgroup when
-// the batch holds at most kFuseMax items.
+// the batch holds at most fuse_max items.
 int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
     hipStream_t s = j->ctx->stream;
     int rc;
     if ((rc = step_begin(j))) return rc;
+    // content-independent work a stage ahead on the side stream when two
+    // buffer sets of the largest stage fit the budget
+    j->ahead_on = j->tune.ahead && ahead_bytes(j) <= kAheadBudget;
+    if (j->ahead_on) {
+        const uint64_t smax = std::max<uint64_t>(1, stage_size(j, 0));
+        for (tsg_job::Ahead& A : j->ahead) {
+            HIP_TRY(A.rc.reserve(smax * j->prm.random_sample_locations));
+            HIP_TRY(A.nbrs.reserve(smax * j->prm.nearest_neighbors));
+            HIP_TRY(A.cnt.reserve(smax));
+            HIP_TRY(A.n_at.reserve(smax));
+            HIP_TRY(A.mend.reserve((size_t)j->tiles_x * j->tiles_y));
+            HIP_TRY(A.mnew.reserve((size_t)j->tiles_x * j->tiles_y));
+            HIP_TRY(A.tmin.reserve((size_t)j->tiles_x * j->tiles_y));
+            if (!A.ready) HIP_TRY(hipEventCreateWithFlags(&A.ready, hipEventDisableTiming));
+        }
+        j->ahead_open = j->n_res;
+        j->ahead_next = 0;
+        // the side stream starts after the set-up work on the job stream
+        hipEvent_t e;
+        HIP_TRY(next_event(j, &e));
+        HIP_TRY(hipEventRecord(e, s));
+        HIP_TRY(hipStreamWaitEvent(j->ctx->side, e, 0));
+        if ((rc = ahead_stage(j))) return rc;
+    }
     int last_pct = 0;
     std::vector<uint8_t> snap;
     std::vector<uint32_t> sb_bounds, sb_nres;
@@ -722,15 +1179,14 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
         if (sb_nres.empty()) return TSG_OK;
         int rc2 = step_ensure_rc(j, sb_bounds.front(), sb_bounds.back());
         if (rc2) return rc2;
-        j->small_keep.emplace_back(sb_bounds);
-        std::vector<uint32_t>& tab = j->small_keep.back();
+        std::vector<uint32_t> tab(sb_bounds);
         tab.insert(tab.end(), sb_nres.begin(), sb_nres.end());
         const size_t nb = sb_nres.size();
         // the table lives in a per-run device arena: bounds (nb+1) then nres (nb)
         if (j->small_used + tab.size() > j->small_tab.n) return fail(TSG_ERR_DEVICE, "small-batch table overflow");
         uint32_t* dt = j->small_tab.p + j->small_used;
         j->small_used += tab.size();
-        HIP_TRY(hipMemcpyAsync(dt, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(j->ctx->staging.upload(dt, tab.data(), tab.size() * 4, s));
         const BatchArgs& a = j->step.a;
         HIP_TRY(timed_launch(j, K_RESOLVE, sb_bounds.back() - sb_bounds.front(), [&] {
             return launch_resolve_small(a, dt, dt + nb + 1, (uint32_t)nb, j->guided, s);
@@ -743,7 +1199,12 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
         if ((rc = step_open_stage(j))) return rc;
         uint64_t i0, i1;
         while (step_peek(j, i0, i1)) {
-            if (i1 - i0 <= small_batch_max()) {
+            if (step_wants_prefetch(j, i0, i1)) {
+                // earlier small batches read the previous window: launch them first
+                if ((rc = flush_small())) return rc;
+                if ((rc = step_prefetch(j, i0))) return rc;
+            }
+            if (i1 - i0 <= j->tune.small_max) {
                 if (sb_bounds.empty()) sb_bounds.push_back((uint32_t)i0);
                 sb_bounds.push_back((uint32_t)i1);
                 sb_nres.push_back((uint32_t)j->n_res);
@@ -754,8 +1215,8 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
                 a.n_res = (uint32_t)j->n_res;
                 a.item0 = (uint32_t)i0;
                 a.item1 = (uint32_t)i1;
-                a.fused_commit = (i1 - i0) <= kFuseMax;
-                a.team = team_for(i1 - i0);
+                a.fused_commit = (i1 - i0) <= j->tune.fuse_max;
+                a.team = team_for(i1 - i0, j->tune.team_max, j->tune.team_waves);
                 HIP_TRY(timed_launch(j, K_RESOLVE, i1 - i0, [&] { return launch_resolve(a, j->guided, s); }));
                 if (!a.fused_commit) {
                     CommitArgs ca{j->W, j->tiles_x, a.maps, a.work, j->result.p, (uint32_t)i0, (uint32_t)i1,
@@ -788,18 +1249,6 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
 
 // ---- multi-job engine ----
 
-// All stages' pick orders of a job (content independent; computed ahead on
-// host threads when the job is admitted to the engine).
-void precompute_picks(tsg_job* j) {
-    const int P = (int)j->prm.backtrack_stages;
-    if (j->picks_keep.size() != (size_t)P + 1) {
-        j->picks_keep.assign(P + 1, {});
-        j->picks_ready.assign(P + 1, 0);
-    }
-    j->step.planned_res = j->n_res;
-    for (int t = 0; t <= P; ++t) stage_picks(j, t);
-}
-
 size_t job_footprint(const tsg_job* j) {
     size_t b = 0;
     for (auto& e : j->ex) b += e->n * 4;
@@ -808,6 +1257,10 @@ size_t job_footprint(const tsg_job* j) {
     b += j->tg.n * 4 + j->maps.n * sizeof(MapDesc);
     b += j->N * (sizeof(uint4) + 4 + 4 + 4) + (size_t)j->tiles_x * j->tiles_y * 8;
     b += j->result.n * sizeof(uint4) + j->rc.n * sizeof(uint2) + j->small_tab.n * 4;
+    b += j->rank.n * 4 + j->mend.n * 8 + j->tmin.n * 4 + j->pre_n.n * 4 + j->pre_cnt.n * 4 + j->pre_nbrs.n * 4;
+    for (const tsg_job::Ahead& A : j->ahead)
+        b += A.rc.n * sizeof(uint2) + A.nbrs.n * 4 + A.cnt.n * 4 + A.n_at.n * 4 + A.mend.n * 8 + A.mnew.n * 8 +
+             A.tmin.n * 4;
     return b;
 }
 
@@ -827,17 +1280,21 @@ struct tsg_batch {
     // per-round tables: units of both kinds, then the workgroup and item starts
     BatchArgs* h_units[kRing] = {};
     uint32_t* h_idx[kRing] = {};
+    RandArgs* h_rand[kRing] = {};
     DBuf<BatchArgs> d_units[kRing];
     DBuf<uint32_t> d_idx[kRing];
+    DBuf<RandArgs> d_rand[kRing];
     hipEvent_t ring_ev[kRing] = {};
     bool ring_used[kRing] = {};
     uint32_t cap_units = 0;
     int ring = 0;
     uint64_t rounds = 0, launches = 0, units_total = 0;
+    tsg_tuning tune{};
     ~tsg_batch() {
         for (int r = 0; r < kRing; ++r) {
             if (h_units[r]) (void)hipHostFree(h_units[r]);
             if (h_idx[r]) (void)hipHostFree(h_idx[r]);
+            if (h_rand[r]) (void)hipHostFree(h_rand[r]);
             if (ring_ev[r]) (void)hipEventDestroy(ring_ev[r]);
         }
     }
@@ -854,15 +1311,8 @@ int batch_admit(tsg_batch* b) {
         adm.push_back(e.job);
     }
     if (adm.empty()) return TSG_OK;
-    // pick orders of the admitted jobs, one host thread per job (<= 16 at a time)
-    std::atomic<size_t> next{0};
-    unsigned nt = std::min<size_t>(adm.size(), 16);
-    std::vector<std::thread> th;
-    for (unsigned t = 0; t < nt; ++t)
-        th.emplace_back([&] {
-            for (size_t q; (q = next.fetch_add(1)) < adm.size();) precompute_picks(adm[q]);
-        });
-    for (auto& x : th) x.join();
+    // pick orders are computed by the context's pool; rounds skip a job
+    // until its next stage's order is ready
     for (tsg_job* j : adm) {
         int rc = step_begin(j);
         if (rc) return rc;
@@ -881,17 +1331,30 @@ int batch_round(tsg_batch* b) {
         uint64_t i0, i1;
     };
     std::vector<U> units[2];
+    std::vector<RandArgs> rands;
+    tsg_job* waiter = nullptr;  // a job whose next pick order is not ready
     for (size_t q = 0; q < b->active.size();) {
         tsg_job* j = b->active[q].job;
         uint64_t i0 = 0, i1 = 0;
-        bool got = false;
+        bool got = false, waiting = false;
         while (!step_done(j)) {
-            if (!j->step.open && (rc = step_open_stage(j))) return rc;
+            if (!j->step.open) {
+                if (!picks_ready_now(j, j->step.t)) {
+                    waiting = true;
+                    break;
+                }
+                if ((rc = step_open_stage(j))) return rc;
+            }
             if (step_peek(j, i0, i1)) {
                 got = true;
                 break;
             }
             if ((rc = step_close_stage(j))) return rc;
+        }
+        if (waiting) {
+            waiter = j;
+            ++q;
+            continue;
         }
         if (!got) {
             if ((rc = step_end(j))) return rc;
@@ -899,12 +1362,20 @@ int batch_round(tsg_batch* b) {
             b->active.erase(b->active.begin() + q);
             continue;
         }
-        if ((rc = step_ensure_rc(j, i0, i1))) return rc;
+        RandArgs ra;
+        bool need_rc;
+        if ((rc = step_plan_rc(j, i0, i1, &ra, &need_rc))) return rc;
+        if (need_rc) rands.push_back(ra);
+        if (step_wants_prefetch(j, i0, i1) && (rc = step_prefetch(j, i0))) return rc;
         units[j->guided ? 1 : 0].push_back({j, i0, i1});
         ++q;
     }
     const uint32_t nu = (uint32_t)(units[0].size() + units[1].size());
-    if (nu == 0) return TSG_OK;
+    if (nu == 0) {
+        // nothing to launch: wait for a pick order instead of spinning
+        if (waiter && (rc = wait_picks(waiter, waiter->step.t))) return rc;
+        return TSG_OK;
+    }
     // ring slot: wait until the round that last used it has finished
     const int r = b->ring;
     b->ring = (b->ring + 1) % tsg_batch::kRing;
@@ -915,10 +1386,13 @@ int batch_round(tsg_batch* b) {
         for (int q = 0; q < tsg_batch::kRing; ++q) {
             if (b->h_units[q]) HIP_TRY(hipHostFree(b->h_units[q]));
             if (b->h_idx[q]) HIP_TRY(hipHostFree(b->h_idx[q]));
+            if (b->h_rand[q]) HIP_TRY(hipHostFree(b->h_rand[q]));
             HIP_TRY(hipHostMalloc((void**)&b->h_units[q], cap * sizeof(BatchArgs)));
-            HIP_TRY(hipHostMalloc((void**)&b->h_idx[q], 4 * (cap + 2) * sizeof(uint32_t)));
+            HIP_TRY(hipHostMalloc((void**)&b->h_idx[q], 5 * (cap + 2) * sizeof(uint32_t)));
+            HIP_TRY(hipHostMalloc((void**)&b->h_rand[q], cap * sizeof(RandArgs)));
             HIP_TRY(b->d_units[q].alloc(cap));
-            HIP_TRY(b->d_idx[q].alloc(4 * (cap + 2)));
+            HIP_TRY(b->d_idx[q].alloc(5 * (cap + 2)));
+            HIP_TRY(b->d_rand[q].alloc(cap));
         }
         b->cap_units = cap;
     }
@@ -932,7 +1406,7 @@ int batch_round(tsg_batch* b) {
     uint64_t round_items = 0;
     for (int kd = 0; kd < 2; ++kd)
         for (const U& u : units[kd]) round_items += u.i1 - u.i0;
-    const uint32_t team = team_for(round_items);
+    const uint32_t team = team_for(round_items, b->tune.team_max, b->tune.team_waves);
     const uint32_t ipw = items_per_wg(team);
     for (int kd = 0; kd < 2; ++kd) {
         for (size_t q = 0; q < units[kd].size(); ++q) {
@@ -952,8 +1426,22 @@ int batch_round(tsg_batch* b) {
         }
         off += (uint32_t)units[kd].size();
     }
+    // random candidates of every job that needs a new chunk, one launch
+    uint32_t* rs = hi + 4 * (b->cap_units + 1);
+    uint32_t n_rand = 0;
+    for (size_t q = 0; q < rands.size(); ++q) {
+        b->h_rand[r][q] = rands[q];
+        rs[q] = n_rand;
+        n_rand += rands[q].item1 - rands[q].item0;
+    }
     HIP_TRY(hipMemcpyAsync(b->d_units[r].p, hu, nu * sizeof(BatchArgs), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(b->d_idx[r].p, hi, 4 * (b->cap_units + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(b->d_idx[r].p, hi, 5 * (b->cap_units + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    if (!rands.empty()) {
+        HIP_TRY(hipMemcpyAsync(b->d_rand[r].p, b->h_rand[r], rands.size() * sizeof(RandArgs), hipMemcpyHostToDevice, s));
+        HIP_TRY(launch_rand_multi(b->d_rand[r].p, b->d_idx[r].p + 4 * (b->cap_units + 1), (uint32_t)rands.size(),
+                                  n_rand, s));
+        b->launches += 1;
+    }
     const BatchArgs* du[2] = {b->d_units[r].p, b->d_units[r].p + units[0].size()};
     const uint32_t* dw[2] = {b->d_idx[r].p, b->d_idx[r].p + (b->cap_units + 1)};
     const uint32_t* di[2] = {b->d_idx[r].p + 2 * (b->cap_units + 1), b->d_idx[r].p + 3 * (b->cap_units + 1)};
@@ -1036,6 +1524,9 @@ int tsg_ctx_create(int device, tsg_ctx** out) {
     std::unique_ptr<tsg_ctx> c(new tsg_ctx());
     c->device = device;
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    HIP_TRY(c->staging.init());
+    HIP_TRY(c->staging_side.init());
     HIP_TRY(configure_kernels());
     const std::vector<int32_t>& tab = offset_table();
     HIP_TRY(c->offtab.alloc(tab.size()));
@@ -1051,9 +1542,11 @@ void tsg_ctx_destroy(tsg_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     ctx->offtab.release();
     ctx->dummy_lut.release();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     delete ctx;
 }
 
@@ -1118,6 +1611,7 @@ int tsg_job_create(tsg_ctx* ctx, const tsg_params* prm, uint32_t n_examples, con
     j->ctx = ctx;
     j->prm = *prm;
     tsg_schedule_default(&j->sched);
+    tsg_tuning_default(&j->tune);
     j->W = out_w;
     j->H = out_h;
     j->N = (size_t)out_w * out_h;
@@ -1211,6 +1705,9 @@ int tsg_job_create(tsg_ctx* ctx, const tsg_params* prm, uint32_t n_examples, con
     HIP_TRY(j->st_sum.alloc(P + 1));
     HIP_TRY(j->st_cnt.alloc(P + 1));
     HIP_TRY(j->done_counter.alloc(1));
+    HIP_TRY(j->rank.alloc(j->N));
+    HIP_TRY(j->mend.alloc((size_t)j->tiles_x * j->tiles_y));
+    HIP_TRY(j->tmin.alloc((size_t)j->tiles_x * j->tiles_y));
     HIP_TRY(hipMemset(j->done_counter.p, 0, 4));
     // Generator::new / new_from_inpaint and the random init are content-
     // independent of the stages: upload the initial state now
@@ -1218,6 +1715,28 @@ int tsg_job_create(tsg_ctx* ctx, const tsg_params* prm, uint32_t n_examples, con
     j->state_ready = true;
     HIP_TRY(hipStreamSynchronize(s));
     *out = j.release();
+    return TSG_OK;
+}
+
+void tsg_tuning_default(tsg_tuning* t) {
+    // measured on the 512^2 -> 1024^2 workload: the last-workgroup commit
+    // (a device-wide fence and counter per workgroup) costs more than a
+    // separate commit launch; the one-workgroup loop pays only while batches
+    // leave most of its 16 waves to teams
+    t->small_max = 4;
+    t->fuse_max = 0;
+    t->team_max = 8;
+    t->team_waves = 4096;
+    t->prefetch_max = 2048;
+    t->ahead = 1;
+}
+
+int tsg_job_set_tuning(tsg_job* job, const tsg_tuning* t) {
+    if (!job || !t) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    if (t->small_max > small_batch_max()) return fail(TSG_ERR_INVALID_ARGUMENT, "small_max above 16");
+    if (t->team_max == 0 || t->team_max > 8 || (t->team_max & (t->team_max - 1)))
+        return fail(TSG_ERR_INVALID_ARGUMENT, "team_max must be 1, 2, 4 or 8");
+    job->tune = *t;
     return TSG_OK;
 }
 
@@ -1239,30 +1758,22 @@ int tsg_job_download(tsg_job* job, uint8_t* rgba, uint32_t* coord, uint32_t* ids
                      float* resolved_score) {
     if (!job) return fail(TSG_ERR_INVALID_ARGUMENT, "null job");
     HIP_TRY(hipSetDevice(job->ctx->device));
-    std::vector<uint4> st(job->N);
-    HIP_TRY(hipMemcpy(st.data(), job->state.p, job->N * sizeof(uint4), hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < job->N; ++i) {
-        if (rgba) memcpy(rgba + i * 4, &st[i].x, 4);
-        if (coord) {
-            coord[i * 3] = st[i].y & 0xffffu;
-            coord[i * 3 + 1] = st[i].y >> 16;
-            coord[i * 3 + 2] = st[i].z & 0xffffu;
-        }
-        if (ids) {
-            ids[i * 2] = st[i].w;
-            ids[i * 2 + 1] = st[i].z >> 16;
-        }
-    }
-    if (resolved_flat || resolved_score) {
-        std::vector<uint32_t> rl(job->n_res);
-        if (job->n_res) HIP_TRY(hipMemcpy(rl.data(), job->rlist.p, job->n_res * 4, hipMemcpyDeviceToHost));
-        std::vector<float> sc(job->N);
-        HIP_TRY(hipMemcpy(sc.data(), job->score.p, job->N * 4, hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < job->n_res; ++i) {
-            if (resolved_flat) resolved_flat[i] = rl[i];
-            if (resolved_score) resolved_score[i] = sc[rl[i]];
-        }
-    }
+    std::lock_guard<std::mutex> lk(job->ctx->mu);
+    hipStream_t s = job->ctx->stream;
+    // unpack on the device (8 + 12 + 8 + 4 bytes per pixel), then one copy per map
+    const size_t N = job->N, R = job->n_res;
+    DBuf<uint32_t> out;
+    HIP_TRY(out.alloc(6 * N + R));
+    UnpackArgs ua{job->state.p, (uint32_t)N, rgba ? out.p : nullptr, coord ? out.p + N : nullptr,
+                  ids ? out.p + 4 * N : nullptr, job->rlist.p, job->score.p, resolved_score ? (uint32_t)R : 0u,
+                  resolved_score ? reinterpret_cast<float*>(out.p + 6 * N) : nullptr};
+    HIP_TRY(launch_unpack(ua, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (rgba) HIP_TRY(hipMemcpy(rgba, out.p, N * 4, hipMemcpyDeviceToHost));
+    if (coord) HIP_TRY(hipMemcpy(coord, out.p + N, N * 12, hipMemcpyDeviceToHost));
+    if (ids) HIP_TRY(hipMemcpy(ids, out.p + 4 * N, N * 8, hipMemcpyDeviceToHost));
+    if (resolved_flat && R) HIP_TRY(hipMemcpy(resolved_flat, job->rlist.p, R * 4, hipMemcpyDeviceToHost));
+    if (resolved_score && R) HIP_TRY(hipMemcpy(resolved_score, out.p + 6 * N, R * 4, hipMemcpyDeviceToHost));
     return TSG_OK;
 }
 
@@ -1311,7 +1822,7 @@ int tsg_job_phase_clocks(tsg_job* job, uint64_t* out, uint64_t n_items) {
 
 int tsg_job_kernel_stats(tsg_job* job, double* ms, uint64_t* items, uint64_t* launches) {
     if (!job) return fail(TSG_ERR_INVALID_ARGUMENT, "null job");
-    for (int q = 0; q < 5; ++q) {
+    for (int q = 0; q < 6; ++q) {
         if (ms) ms[q] = job->kern_ms[q];
         if (items) items[q] = job->kern_items[q];
         if (launches) launches[q] = job->kern_launches[q];
@@ -1657,6 +2168,9 @@ int tsg_footprint_estimate(const tsg_params* prm, uint32_t n_examples, const uin
     const uint64_t per_seg = (uint64_t)(small_batch_max() + 1) << 6;
     b += (3 * std::min<uint64_t>(items, 2 * (P + 1) * per_seg) + 4 * (P + 1) + 16) * 4;
     b += (P + 1) * 16 + 512 * 4 + 64;
+    // rank map, window mask, prefetched neighbour lists
+    const uint64_t win = std::min<uint64_t>(N, kPrefetchCap);
+    b += N * 4 + (uint64_t)((out_w + 7) / 8) * ((out_h + 7) / 8) * 12 + win * (8 + 4ull * prm->nearest_neighbors);
     *bytes = b;
     return TSG_OK;
 }
@@ -1668,6 +2182,7 @@ int tsg_batch_create(tsg_ctx* ctx, uint32_t max_active, tsg_batch** out) {
     std::unique_ptr<tsg_batch> b(new tsg_batch());
     b->ctx = ctx;
     b->max_active = max_active;
+    tsg_tuning_default(&b->tune);
     for (int r = 0; r < tsg_batch::kRing; ++r) HIP_TRY(hipEventCreateWithFlags(&b->ring_ev[r], hipEventDisableTiming));
     *out = b.release();
     return TSG_OK;
@@ -1678,6 +2193,7 @@ int tsg_batch_submit(tsg_batch* b, tsg_job* job, uint64_t* ticket) {
     if (job->ctx != b->ctx) return fail(TSG_ERR_INVALID_ARGUMENT, "job belongs to another context");
     if (job->ran || job->step.active) return fail(TSG_ERR_INVALID_ARGUMENT, "a job runs once");
     job->rc_budget = kEngineRcBudget;
+    job->pk_threads = 2;
     *ticket = b->next_ticket++;
     b->pending.push_back({*ticket, job, 0});
     return TSG_OK;

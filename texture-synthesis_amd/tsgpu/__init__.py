@@ -70,6 +70,23 @@ class Schedule(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [
+        ("small_max", C.c_uint32),
+        ("fuse_max", C.c_uint32),
+        ("team_max", C.c_uint32),
+        ("team_waves", C.c_uint32),
+        ("prefetch_max", C.c_uint32),
+        ("ahead", C.c_uint32),
+    ]
+
+
+def default_tuning() -> dict:
+    t = Tuning()
+    lib().tsg_tuning_default(C.byref(t))
+    return {n: getattr(t, n) for n, _ in Tuning._fields_}
+
+
 class Image(C.Structure):
     _fields_ = [("w", C.c_uint32), ("h", C.c_uint32), ("rgba", C.c_void_p)]
 
@@ -98,6 +115,8 @@ def lib():
     L.tsg_job_create.argtypes = [vp, P(Params), u32, P(Image), P(C.c_int), P(Image), P(Image), P(Image), P(Image),
                                  u32, u32, u32, P(vp)]
     L.tsg_job_set_schedule.argtypes = [vp, P(Schedule)]
+    L.tsg_tuning_default.argtypes = [P(Tuning)]
+    L.tsg_job_set_tuning.argtypes = [vp, P(Tuning)]
     L.tsg_job_run.argtypes = [vp, PROGRESS_FN, vp]
     L.tsg_job_download.argtypes = [vp, vp, vp, vp, vp, vp]
     L.tsg_job_stage_stats.argtypes = [vp, vp, vp, vp, u32]
@@ -235,7 +254,7 @@ class Example:
 class GeneratedImage:
     """GeneratedImage (lib.rs:378-455): image plus coord / id maps."""
 
-    KERNELS = ("resolve", "commit", "rand", "recolor", "stats")
+    KERNELS = ("resolve", "commit", "rand", "recolor", "stats", "prefetch")
 
     def __init__(self, image, coord, ids, resolved_flat, resolved_score, stage_stats, run_ms, n_batches,
                  original_maps, kernel_stats=None):
@@ -327,6 +346,16 @@ class Session:
         _check(lib().tsg_job_set_schedule(self._job, C.byref(s)))
         return self
 
+    def set_tuning(self, **kw) -> "Session":
+        """Launch structure (results do not depend on it); keys of default_tuning()."""
+        t = default_tuning()
+        unknown = set(kw) - set(t)
+        if unknown:
+            raise TypeError(f"unknown tuning keys {sorted(unknown)}")
+        t.update(kw)
+        _check(lib().tsg_job_set_tuning(self._job, C.byref(Tuning(**t))))
+        return self
+
     def set_profiling(self, level: int = 1) -> "Session":
         _check(lib().tsg_job_set_profiling(self._job, int(level)))
         self._profiled = level
@@ -360,9 +389,9 @@ class Session:
         """Downloads the finished job's maps, statistics and timings."""
         L = lib()
         w, h = self._out.width, self._out.height
-        img = np.zeros((h, w, 4), np.uint8)
-        coord = np.zeros((h * w, 3), np.uint32)
-        ids = np.zeros((h * w, 2), np.uint32)
+        img = np.empty((h, w, 4), np.uint8)  # written whole by the download
+        coord = np.empty((h * w, 3), np.uint32)
+        ids = np.empty((h * w, 2), np.uint32)
         rf = np.zeros(h * w, np.uint32)
         rs = np.zeros(h * w, np.float32)
         _check(L.tsg_job_download(self._job, _ptr(img), _ptr(coord), _ptr(ids), _ptr(rf), _ptr(rs)))
@@ -378,9 +407,9 @@ class Session:
         _check(L.tsg_job_timing(self._job, C.byref(ms), C.byref(nb)))
         kst = None
         if getattr(self, "_profiled", False):
-            kms = np.zeros(5, np.float64)
-            kit = np.zeros(5, np.uint64)
-            kla = np.zeros(5, np.uint64)
+            kms = np.zeros(6, np.float64)
+            kit = np.zeros(6, np.uint64)
+            kla = np.zeros(6, np.uint64)
             _check(L.tsg_job_kernel_stats(self._job, _ptr(kms), _ptr(kit), _ptr(kla)))
             kst = {n: dict(ms=float(kms[i]), items=int(kit[i]), launches=int(kla[i]))
                    for i, n in enumerate(GeneratedImage.KERNELS)}
