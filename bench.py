@@ -173,11 +173,32 @@ def serving_leg(ts, synth, n, qps, budget_gib):
 
 
 def traffic_from_profiles():
-    path = os.path.join(ROOT, "profiles", "resolve_c2_traffic.json")
+    """HBM bytes per k_resolve launch of this workload, from the committed
+    counter summary (FETCH_SIZE + WRITE_SIZE per dispatch, copy-calibrated)."""
+    import re
+    path = os.path.join(ROOT, "profiles", "resolve_c2_traffic.md")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        m = re.search(r"^hbm_bytes_per_launch:\s*([0-9.]+)", f.read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def copy_calibration(mb):
+    """A plain device-to-device copy of `mb` MiB (runtime copy kernel), the
+    reference point that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mb << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": n, "copies": 3}))
 
 
 def main():
@@ -191,7 +212,12 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=120, help="requests in the serving leg (0 = skip)")
     ap.add_argument("--serve-qps", type=float, default=12.0)
     ap.add_argument("--serve-budget-gib", type=float, default=4.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X; --gpus must be 1", file=sys.stderr)
         sys.exit(2)

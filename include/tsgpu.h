@@ -83,7 +83,12 @@ typedef struct tsg_schedule {
  *   ahead        1: a single-job run prepares each stage's content-independent
  *                work (random candidates, pick ranks, every neighbour list)
  *                during the previous stage on a second stream, when it fits
- *                in device memory; 0: per window on the job stream */
+ *                in device memory; 0: per window on the job stream
+ *   coop_wgs     workgroups (16 waves each) of the cooperative launch that
+ *                runs consecutive latency-bound batches with grid barriers
+ *                instead of two launches per batch (0 = off)
+ *   coop_max     batches of at most this many items (<= 16 * coop_wgs) go
+ *                to that launch */
 typedef struct tsg_tuning {
     uint32_t small_max;
     uint32_t fuse_max;
@@ -91,6 +96,8 @@ typedef struct tsg_tuning {
     uint32_t team_waves;
     uint32_t prefetch_max;
     uint32_t ahead;
+    uint32_t coop_wgs;
+    uint32_t coop_max;
 } tsg_tuning;
 
 typedef struct tsg_image {
@@ -155,15 +162,15 @@ int tsg_job_stage_stats(tsg_job* job, double* sums, uint64_t* counts, uint64_t* 
 /* Device time of the last tsg_job_run in milliseconds, and batch count. */
 int tsg_job_timing(tsg_job* job, double* run_ms, uint64_t* n_batches);
 /* Per-kernel device timing of the next tsg_job_run (HIP events on the job's
- * stream around every launch). Level 2 also records, for each work item of
- * the first stage, device timestamps (100 MHz wall clock) at the phase
- * boundaries of its resolve: start, kNN, weights, coherence dedup,
- * coherence costs, random costs, end. */
+ * stream around every launch). Level L >= 2 also records, for each work item
+ * of stage L - 2 (0 = the first), device timestamps (100 MHz wall clock) at
+ * the phase boundaries of its resolve: start, kNN, neighbour state and
+ * coherence candidates, first candidate group, all groups, end. */
 int tsg_job_set_profiling(tsg_job* job, int level);
-/* Phase record of the first n_items work items of the first stage (16 u64
- * each) after a level-2 profiled run: [0..6] the phase timestamps above,
- * [7] kNN search done (in-kernel searches only), [8] ring-search steps,
- * [9] keys buffered before the final selection. */
+/* Phase record of the first n_items work items of the recorded stage (16 u64
+ * each) after a level >= 2 profiled run: [0..6] the timestamps above (slot 3
+ * unused), [7] kNN search done (in-kernel searches only), [8] ring-search
+ * steps, [9] keys buffered before the final selection. */
 int tsg_job_phase_clocks(tsg_job* job, uint64_t* out, uint64_t n_items);
 /* After a profiled run, per kernel [resolve, commit, random candidates,
  * recolour, stats, kNN prefetch]: summed milliseconds, items processed,

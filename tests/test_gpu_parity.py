@@ -231,10 +231,13 @@ def test_default_schedule_bit_exact_c1(ctx, seed):
 
 
 @pytest.mark.parametrize("tuning", [
-    dict(small_max=0, fuse_max=0, team_max=1, prefetch_max=0, ahead=0),
-    dict(small_max=4, fuse_max=64, team_max=2, prefetch_max=64, ahead=0),
-    dict(small_max=16, fuse_max=100000, team_max=8, team_waves=1 << 20, prefetch_max=100000, ahead=0),
-    dict(small_max=16, fuse_max=2048, team_max=4, ahead=1),
+    dict(small_max=0, fuse_max=0, team_max=1, prefetch_max=0, ahead=0, coop_wgs=0, coop_max=0),
+    dict(small_max=4, fuse_max=64, team_max=2, prefetch_max=64, ahead=0, coop_wgs=0, coop_max=0),
+    dict(small_max=16, fuse_max=100000, team_max=8, team_waves=1 << 20, prefetch_max=100000, ahead=0,
+         coop_wgs=0, coop_max=0),
+    dict(small_max=16, fuse_max=2048, team_max=4, ahead=1, coop_wgs=0, coop_max=0),
+    dict(coop_wgs=8, coop_max=128, ahead=0),
+    dict(coop_wgs=256, coop_max=4096, team_max=8),
 ])
 def test_launch_tuning_does_not_change_results(ctx, tuning):
     # every launch structure resolves the same batches against the same states

@@ -161,6 +161,11 @@ hipError_t launch_resolve_multi(const BatchArgs* units, const uint32_t* wg_start
 hipError_t launch_commit_multi(const BatchArgs* units, const uint32_t* item_start, uint32_t n_units, uint32_t total,
                                hipStream_t s);
 // consecutive batches of at most small_batch_max() items, one workgroup
+// Consecutive batches (each at most wgs * 16 items) in one cooperative launch
+// of wgs workgroups of 16 waves; wgs <= coop_max_wgs(guided).
+uint32_t coop_max_wgs(bool guided);
+hipError_t launch_resolve_coop(const BatchArgs& a, const uint32_t* bounds, const uint32_t* nres, uint32_t n_batches,
+                               uint32_t wgs, uint32_t team_max, uint32_t* overflow, bool guided, hipStream_t s);
 hipError_t launch_resolve_small(const BatchArgs& a, const uint32_t* bounds, const uint32_t* nres,
                                 uint32_t n_batches, bool guided, hipStream_t s);
 hipError_t launch_resolve(const BatchArgs& a, bool guided, hipStream_t s);

@@ -14,6 +14,7 @@
 // terms (then four guide terms) into one accumulator, then score += s * w as
 // a separate multiply and add. Build with -ffp-contract=off.
 #include <float.h>
+#include <hip/hip_cooperative_groups.h>
 #include <math.h>
 
 #include "kernels.h"
@@ -374,26 +375,34 @@ __device__ __forceinline__ void knn_rings(const KnnView& a, WaveLds* L, int x, i
 
 // Ring-ordered offset table scan: the first k resolved pixels in table
 // order are the k nearest (the table lists every offset within its radius
-// in canonical order). Returns the number found (<= k).
+// in canonical order). 256 offsets per memory round trip, taken in table
+// order. Returns the number found (<= k).
 __device__ __forceinline__ uint32_t knn_table(const KnnView& a, WaveLds* L, int x, int y, uint32_t k, int lane) {
     uint32_t found = 0;
-    for (uint32_t base = 0; base < a.offtab_n; base += kWave) {
-        uint32_t t = base + lane;
-        bool hit = false;
-        int32_t o = 0;
-        if (t < a.offtab_n) {
-            o = a.offtab[t];
-            int X = x + packed_dx(o), Y = y + packed_dy(o);
-            const bool in = X >= 0 && Y >= 0 && X < (int)a.W && Y < (int)a.H;
-            hit = in && px_set(a.mask, a.tiles_x, X, Y);
-            if (in && !hit && a.mask_new && px_set(a.mask_new, a.tiles_x, X, Y))
-                hit = a.rank[(uint32_t)Y * a.W + (uint32_t)X] < a.n_at;
+    for (uint32_t base = 0; base < a.offtab_n && found < k; base += 4 * kWave) {
+        int32_t o[4];
+        bool hit[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t t = base + u * kWave + lane;
+            o[u] = 0;
+            hit[u] = false;
+            if (t < a.offtab_n) {
+                o[u] = a.offtab[t];
+                const int X = x + packed_dx(o[u]), Y = y + packed_dy(o[u]);
+                const bool in = X >= 0 && Y >= 0 && X < (int)a.W && Y < (int)a.H;
+                hit[u] = in && px_set(a.mask, a.tiles_x, X, Y);
+                if (in && !hit[u] && a.mask_new && px_set(a.mask_new, a.tiles_x, X, Y))
+                    hit[u] = a.rank[(uint32_t)Y * a.W + (uint32_t)X] < a.n_at;
+            }
         }
-        uint64_t m = __ballot(hit);
-        uint32_t pre = (uint32_t)__popcll(m & lanemask_lt(lane));
-        if (hit && found + pre < k) L->nd[found + pre] = o;
-        found += (uint32_t)__popcll(m);
-        if (found >= k) break;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint64_t m = __ballot(hit[u]);
+            const uint32_t pre = (uint32_t)__popcll(m & lanemask_lt(lane));
+            if (hit[u] && found + pre < k) L->nd[found + pre] = o[u];
+            found += (uint32_t)__popcll(m);
+        }
     }
     return found < k ? found : k;
 }
@@ -404,9 +413,9 @@ __device__ __forceinline__ uint32_t knn_search(const KnnView& v, WaveLds* L, int
                                                int lane, uint32_t& iters, uint32_t& buffered) {
     uint32_t found = 0;
     bool use_sel = n_res <= kBruteMax;
-    // the offset table pays off when about k / density <= 256 offsets reach
-    // k resolved pixels; sparser masks use the tile ring search
-    const bool sparse = (uint64_t)n_res * 256ull < (uint64_t)k * v.W * v.H;
+    // the offset table pays off when about k / density <= 1024 offsets (four
+    // round trips) reach k resolved pixels; sparser masks use the ring search
+    const bool sparse = (uint64_t)n_res * 1024ull < (uint64_t)k * v.W * v.H;
     if (!use_sel && !sparse) {
         found = knn_table(v, L, x, y, k, lane);
         use_sel = found < k && found < n_res;
@@ -898,9 +907,9 @@ __device__ void resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t i
         exact = a.exact || a.ext_full_cost || a.lut_nan || !(L->avg > 0.0);
         if (!exact) {
             // ---- 3. this member's slice, exact pruning ----
-            // slice: unique coherence candidates [c0, c0 + ncs) then random ones [r0, r0 + nrs).
-            // One member: the coherence group alone first (it usually holds the
-            // winner and bounds the rest), then the random ones 64 at a time.
+            // slice: unique coherence candidates [c0, c0 + ncs) then random ones
+            // [r0, r0 + nrs): the coherence group alone first (it usually holds
+            // the winner and bounds the rest), then the random ones 64 at a time.
             const uint32_t n_u = (uint32_t)__popcll(uniq);
             const uint32_t cper = (n_u + (uint32_t)T - 1) / (uint32_t)T;
             const uint32_t c0 = min(n_u, (uint32_t)member * cper);
@@ -912,7 +921,7 @@ __device__ void resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t i
             uint32_t g_lo = 0;
             bool first = true;
             while (g_lo < nv) {
-                const uint32_t g_hi = (T == 1 && first && ncs > 0) ? ncs : min(nv, g_lo + (uint32_t)kWave);
+                const uint32_t g_hi = (first && ncs > 0) ? ncs : min(nv, g_lo + (uint32_t)kWave);
                 // coherence entries of the group
                 if (mine_u) {
                     const uint32_t v = upos - c0;
@@ -1077,6 +1086,45 @@ __global__ __launch_bounds__(kWave* kSmallWaves) void k_resolve_small(BatchArgs 
         if (ci < i1) commit_one(a.W, a.tiles_x, maps, a.work, a.result, ci, i0, a.first_new, a.state_w, a.mask_w,
                                 a.score_w);
         __syncthreads();
+    }
+}
+
+// A run of consecutive latency-bound batches in one cooperative launch (all
+// workgroups resident): per batch, resolve (teams of waves sized to the
+// grid), grid barrier, commit spread over the grid, grid barrier. Every
+// workgroup takes the same 2 * n_batches barriers. A batch larger than the
+// grid holds (one item per wave) sets *overflow and is skipped; the host
+// never forms one.
+template <bool GUIDED>
+__global__ __launch_bounds__(kWave* kSmallWaves) void k_resolve_coop(BatchArgs a, const uint32_t* bounds,
+                                                                     const uint32_t* nres, uint32_t n_batches,
+                                                                     uint32_t team_max, uint32_t* overflow) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ MapDesc smaps[kMapCache];
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    float *lut, *glut;
+    const MapDesc* maps;
+    WaveLds* wls = setup_lds<GUIDED>(a, smem, lut, glut, smaps, maps);
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    const uint32_t waves = gridDim.x * (uint32_t)kSmallWaves;
+    for (uint32_t b = 0; b < n_batches; ++b) {
+        const uint32_t i0 = bounds[b], i1 = bounds[b + 1], n = i1 - i0;
+        if (n <= waves) {
+            int T = (int)min(team_max, (uint32_t)kSmallWaves);
+            while (T > 1 && (uint32_t)T * n > waves) T >>= 1;
+            const int team = wave / T, member = wave % T;
+            const uint32_t item = i0 + blockIdx.x * (uint32_t)(kSmallWaves / T) + (uint32_t)team;
+            resolve_team<GUIDED>(a, maps, item, i0, nres[b], wls + team * T, member, T, item < i1, lut, glut, lane);
+        } else if (blockIdx.x == 0 && threadIdx.x == 0) {
+            atomicOr(overflow, 2u);
+        }
+        grid.sync();
+        if (n <= waves)
+            for (uint32_t ci = i0 + blockIdx.x * blockDim.x + threadIdx.x; ci < i1; ci += gridDim.x * blockDim.x)
+                commit_one(a.W, a.tiles_x, maps, a.work, a.result, ci, i0, a.first_new, a.state_w, a.mask_w,
+                           a.score_w);
+        grid.sync();
     }
 }
 
@@ -1316,6 +1364,12 @@ hipError_t configure_kernels() {
     e = hipFuncSetAttribute((const void*)k_resolve_multi<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)resolve_lds_bytes(false));
     if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)k_resolve_coop<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)resolve_small_lds_bytes(true));
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)k_resolve_coop<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)resolve_small_lds_bytes(false));
+    if (e != hipSuccess) return e;
     return hipFuncSetAttribute((const void*)k_resolve_small<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)resolve_small_lds_bytes(false));
 }
@@ -1397,6 +1451,28 @@ hipError_t launch_commit_multi(const BatchArgs* units, const uint32_t* item_star
     if (n_units == 0 || total == 0) return hipSuccess;
     hipLaunchKernelGGL(k_commit_multi, dim3((total + 255) / 256), dim3(256), 0, s, units, item_start, n_units, total);
     return hipGetLastError();
+}
+
+uint32_t coop_max_wgs(bool guided) {
+    int per_cu = 0, cus = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    const void* f = guided ? (const void*)k_resolve_coop<true> : (const void*)k_resolve_coop<false>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kWave * kSmallWaves,
+                                                     resolve_small_lds_bytes(guided)) != hipSuccess)
+        return 0;
+    return (uint32_t)(per_cu * cus);
+}
+
+hipError_t launch_resolve_coop(const BatchArgs& a, const uint32_t* bounds, const uint32_t* nres, uint32_t n_batches,
+                               uint32_t wgs, uint32_t team_max, uint32_t* overflow, bool guided, hipStream_t s) {
+    if (n_batches == 0) return hipSuccess;
+    if (wgs == 0 || wgs > coop_max_wgs(guided)) return hipErrorCooperativeLaunchTooLarge;
+    BatchArgs args = a;
+    void* params[] = {&args, &bounds, &nres, &n_batches, &team_max, &overflow};
+    const void* f = guided ? (const void*)k_resolve_coop<true> : (const void*)k_resolve_coop<false>;
+    return hipLaunchCooperativeKernel(f, dim3(wgs), dim3(kWave * kSmallWaves), params,
+                                      (unsigned)resolve_small_lds_bytes(guided), s);
 }
 
 hipError_t launch_resolve_small(const BatchArgs& a, const uint32_t* bounds, const uint32_t* nres,

@@ -413,6 +413,7 @@ struct tsg_job {
         uint64_t n_open = 0, S_eff = 0, w_end = 0;
     } ahead[2];
     bool ahead_on = false;
+    bool in_engine = false;   // run by a tsg_batch (no single-job run structures)
     uint64_t ahead_open = 0;  // resolved count at the open of the next stage to prepare
     int ahead_next = 0;       // next stage to prepare
     DBuf<float> item_score;
@@ -851,12 +852,13 @@ int step_begin(tsg_job* j) {
     uint64_t items = 0;
     for (int t = 0; t <= P; ++t) items += stage_size(j, P - t);
     uint64_t bound = items;
-    if (j->sched.px_per_batch == 0 || j->sched.px_per_batch > small_batch_max()) {
+    const uint64_t run_max = std::max<uint64_t>(j->tune.small_max, j->tune.coop_wgs ? j->tune.coop_max : 0);
+    if (j->sched.px_per_batch == 0 || j->sched.px_per_batch > run_max) {
         uint32_t sh = std::max(j->sched.redo_shift, j->sched.new_shift);
-        uint64_t per_seg = sh >= 40 ? items : (uint64_t)(small_batch_max() + 1) << sh;
+        uint64_t per_seg = sh >= 40 ? items : (run_max + 1) << sh;
         bound = std::min<uint64_t>(items, 2ull * (P + 1) * per_seg);
     }
-    HIP_TRY(j->small_tab.reserve(3 * bound + 4 * (P + 1) + 16));
+    if (!j->in_engine) HIP_TRY(j->small_tab.reserve(3 * bound + 4 * (P + 1) + 16));
     // largest batch: a fraction of the resolved count, a fixed size, or the cap
     uint64_t maxb = j->sched.px_per_batch ? j->sched.px_per_batch
                                           : (uint64_t)j->N >> std::min(j->sched.redo_shift, j->sched.new_shift);
@@ -948,7 +950,7 @@ int step_open_stage(tsg_job* j) {
     a.lut_nan = lut_nan;
     a.first_new = (uint32_t)st.redo_eff;
     a.done_counter = j->done_counter.p;
-    a.dbg_clk = (j->profile_level >= 2 && t == 0) ? j->clk.p : nullptr;
+    a.dbg_clk = (j->profile_level >= 2 && t == j->profile_level - 2) ? j->clk.p : nullptr;
     a.state_w = j->state.p;
     a.mask_w = j->mask.p;
     a.score_w = j->score.p;
@@ -1137,6 +1139,7 @@ int step_finish(tsg_job* j) {
     HIP_TRY(hipMemcpy(&ovf, j->overflow.p, 4, hipMemcpyDeviceToHost));
     j->step.active = false;
     j->ran = true;
+    if (ovf & 2u) return fail(TSG_ERR_DEVICE, "a batch exceeded its cooperative launch");
     if (ovf) return fail(TSG_ERR_SAMPLING, "a random-candidate rejection loop hit its bound (sample mask too sparse)");
     return TSG_OK;
 }
@@ -1175,6 +1178,9 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
     int last_pct = 0;
     std::vector<uint8_t> snap;
     std::vector<uint32_t> sb_bounds, sb_nres;
+    bool sb_coop = false;  // the pending run goes to the cooperative kernel
+    const uint32_t coop_wgs = std::min(j->tune.coop_wgs, j->tune.coop_wgs ? coop_max_wgs(j->guided) : 0u);
+    const uint64_t coop_max = std::min<uint64_t>(j->tune.coop_max, (uint64_t)coop_wgs * small_batch_max());
     auto flush_small = [&]() -> int {
         if (sb_nres.empty()) return TSG_OK;
         int rc2 = step_ensure_rc(j, sb_bounds.front(), sb_bounds.back());
@@ -1188,9 +1194,17 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
         j->small_used += tab.size();
         HIP_TRY(j->ctx->staging.upload(dt, tab.data(), tab.size() * 4, s));
         const BatchArgs& a = j->step.a;
-        HIP_TRY(timed_launch(j, K_RESOLVE, sb_bounds.back() - sb_bounds.front(), [&] {
-            return launch_resolve_small(a, dt, dt + nb + 1, (uint32_t)nb, j->guided, s);
-        }));
+        if (sb_coop) {
+            HIP_TRY(timed_launch(j, K_RESOLVE, sb_bounds.back() - sb_bounds.front(), [&] {
+                return launch_resolve_coop(a, dt, dt + nb + 1, (uint32_t)nb, coop_wgs,
+                                           j->tune.team_max >= 8 ? 16u : j->tune.team_max, j->overflow.p,
+                                           j->guided, s);
+            }));
+        } else {
+            HIP_TRY(timed_launch(j, K_RESOLVE, sb_bounds.back() - sb_bounds.front(), [&] {
+                return launch_resolve_small(a, dt, dt + nb + 1, (uint32_t)nb, j->guided, s);
+            }));
+        }
         sb_bounds.clear();
         sb_nres.clear();
         return TSG_OK;
@@ -1204,7 +1218,10 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
                 if ((rc = flush_small())) return rc;
                 if ((rc = step_prefetch(j, i0))) return rc;
             }
-            if (i1 - i0 <= j->tune.small_max) {
+            const bool coop = coop_wgs > 0 && i1 - i0 <= coop_max;
+            if (coop || i1 - i0 <= j->tune.small_max) {
+                if (!sb_nres.empty() && sb_coop != coop && (rc = flush_small())) return rc;
+                sb_coop = coop;
                 if (sb_bounds.empty()) sb_bounds.push_back((uint32_t)i0);
                 sb_bounds.push_back((uint32_t)i1);
                 sb_nres.push_back((uint32_t)j->n_res);
@@ -1729,6 +1746,8 @@ void tsg_tuning_default(tsg_tuning* t) {
     t->team_waves = 4096;
     t->prefetch_max = 2048;
     t->ahead = 1;
+    t->coop_wgs = 0;  // a grid barrier costs more than the launches it saves (measured)
+    t->coop_max = 0;
 }
 
 int tsg_job_set_tuning(tsg_job* job, const tsg_tuning* t) {
@@ -1736,6 +1755,8 @@ int tsg_job_set_tuning(tsg_job* job, const tsg_tuning* t) {
     if (t->small_max > small_batch_max()) return fail(TSG_ERR_INVALID_ARGUMENT, "small_max above 16");
     if (t->team_max == 0 || t->team_max > 8 || (t->team_max & (t->team_max - 1)))
         return fail(TSG_ERR_INVALID_ARGUMENT, "team_max must be 1, 2, 4 or 8");
+    if (t->coop_max > t->coop_wgs * small_batch_max())
+        return fail(TSG_ERR_INVALID_ARGUMENT, "coop_max above 16 items per cooperative workgroup");
     job->tune = *t;
     return TSG_OK;
 }
@@ -2155,18 +2176,10 @@ int tsg_footprint_estimate(const tsg_params* prm, uint32_t n_examples, const uin
     // state, score, resolved list, per-item scores, resolved-mask tiles
     b += N * (sizeof(uint4) + 12) + (uint64_t)((out_w + 7) / 8) * ((out_h + 7) / 8) * 8;
     // scratch under the engine: largest batch results, one random-candidate
-    // chunk, the small-batch tables, per-stage statistics
-    // (default schedule: largest batch N >> 5, small batches for 17 << 6 items per segment)
+    // chunk, per-stage statistics (default schedule: largest batch N >> 5)
     const uint64_t maxb = std::min<uint64_t>(N, std::max<uint64_t>((N >> 5) + 1, small_batch_max()));
     b += maxb * sizeof(uint4);
     b += std::min<uint64_t>(N, std::max<uint64_t>(maxb, kEngineRcBudget / (8 * m))) * m * sizeof(uint2);
-    uint64_t items = 0;
-    for (uint64_t t = 0; t <= P; ++t) {
-        float v = std::pow(prm->backtrack_percent, (float)t) * (float)N;
-        items += v > 0.0f ? (uint64_t)v : 0;
-    }
-    const uint64_t per_seg = (uint64_t)(small_batch_max() + 1) << 6;
-    b += (3 * std::min<uint64_t>(items, 2 * (P + 1) * per_seg) + 4 * (P + 1) + 16) * 4;
     b += (P + 1) * 16 + 512 * 4 + 64;
     // rank map, window mask, prefetched neighbour lists
     const uint64_t win = std::min<uint64_t>(N, kPrefetchCap);
@@ -2193,6 +2206,7 @@ int tsg_batch_submit(tsg_batch* b, tsg_job* job, uint64_t* ticket) {
     if (job->ctx != b->ctx) return fail(TSG_ERR_INVALID_ARGUMENT, "job belongs to another context");
     if (job->ran || job->step.active) return fail(TSG_ERR_INVALID_ARGUMENT, "a job runs once");
     job->rc_budget = kEngineRcBudget;
+    job->in_engine = true;
     job->pk_threads = 2;
     *ticket = b->next_ticket++;
     b->pending.push_back({*ticket, job, 0});

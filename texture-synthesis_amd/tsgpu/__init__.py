@@ -78,6 +78,8 @@ class Tuning(C.Structure):
         ("team_waves", C.c_uint32),
         ("prefetch_max", C.c_uint32),
         ("ahead", C.c_uint32),
+        ("coop_wgs", C.c_uint32),
+        ("coop_max", C.c_uint32),
     ]
 
 
