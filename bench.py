@@ -87,18 +87,22 @@ def batched_leg(ts, synth, n):
                 .output_size(ts.Dims.square(o)) for x, (e, o, k, sd) in zip(exs, specs)]
 
     px = sum(o * o for _, o, _, _ in specs)
+    # timed: the syntheses and their downloads; the result digests after
     sess = [b.build() for b in builders()]
     t0 = time.perf_counter()
-    seq = [digest(s.run()) for s in sess]
+    res = [s.run() for s in sess]
     t_seq = time.perf_counter() - t0
-    del sess
+    seq = [digest(g) for g in res]
+    del sess, res
     sess = [b.build() for b in builders()]
     batch = ts.Batch(ts.Context.default(), max_active=n)
     t0 = time.perf_counter()
     tickets = [batch.submit(s) for s in sess]
     batch.run_all()
-    got = [digest(batch.take(t)) for t in tickets]
+    res = [batch.take(t) for t in tickets]
     t_bat = time.perf_counter() - t0
+    got = [digest(g) for g in res]
+    del res
     st = batch.stats()
     batch.close()
     return {

@@ -153,6 +153,11 @@ int tsg_job_run(tsg_job* job, tsg_progress_fn progress, void* user);
  * be NULL. rgba: w*h*4. coord: w*h*3 u32 (x, y, map) — the .xform body.
  * ids: w*h*2 u32 (patch id, map id). resolved_flat / resolved_score: the
  * resolved list in order (w*h entries). */
+/* Compact form of tsg_job_download: the packed per-pixel state, four u32
+ * per pixel {rgba (R in the low byte), x | y << 16, coord-map id |
+ * id-map id << 16, patch id}, plus the resolved list and its scores (each
+ * pointer may be NULL). */
+int tsg_job_download_packed(tsg_job* job, uint32_t* state, uint32_t* resolved_flat, float* resolved_score);
 int tsg_job_download(tsg_job* job, uint8_t* rgba, uint32_t* coord, uint32_t* ids, uint32_t* resolved_flat,
                      float* resolved_score);
 /* Per stage (index 0 = p_stage P): sum of the winning costs of

@@ -1142,12 +1142,14 @@ __device__ __forceinline__ uint32_t find_unit(const uint32_t* start, uint32_t n,
 // One round of the multi-job engine: unit u is one job's current batch;
 // its workgroups are [wg_start[u], wg_start[u + 1]).
 template <bool GUIDED>
-__global__ __launch_bounds__(kWave* kWavesPerWG, 4) void k_resolve_multi(const BatchArgs* units, const uint32_t* wg_start,
+__global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve_multi(const BatchArgs* __restrict__ units,
+                                                                      const uint32_t* __restrict__ wg_start,
                                                                       uint32_t n_units) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ MapDesc smaps[kMapCache];
-    const uint32_t u = find_unit(wg_start, n_units, blockIdx.x);
-    const BatchArgs& a = units[u];
+    // the unit is uniform across the workgroup: keep its arguments scalar
+    const uint32_t u = (uint32_t)__builtin_amdgcn_readfirstlane((int)find_unit(wg_start, n_units, blockIdx.x));
+    const BatchArgs a = units[u];
     float *lut, *glut;
     const MapDesc* maps;
     WaveLds* wls = setup_lds<GUIDED>(a, smem, lut, glut, smaps, maps);

@@ -1294,13 +1294,11 @@ struct tsg_batch {
     std::vector<Entry> active, pending, done, taken;
     uint64_t next_ticket = 1;
     static constexpr int kRing = 8;
-    // per-round tables: units of both kinds, then the workgroup and item starts
-    BatchArgs* h_units[kRing] = {};
-    uint32_t* h_idx[kRing] = {};
-    RandArgs* h_rand[kRing] = {};
-    DBuf<BatchArgs> d_units[kRing];
-    DBuf<uint32_t> d_idx[kRing];
-    DBuf<RandArgs> d_rand[kRing];
+    // per-round tables, one pinned block per ring slot copied in one piece:
+    // [starts: 5 x (cap + 2) u32 | random-candidate units | batch units]
+    char* h_blk[kRing] = {};
+    DBuf<char> d_blk[kRing];
+    size_t off_rand = 0, off_units = 0;
     hipEvent_t ring_ev[kRing] = {};
     bool ring_used[kRing] = {};
     uint32_t cap_units = 0;
@@ -1309,9 +1307,7 @@ struct tsg_batch {
     tsg_tuning tune{};
     ~tsg_batch() {
         for (int r = 0; r < kRing; ++r) {
-            if (h_units[r]) (void)hipHostFree(h_units[r]);
-            if (h_idx[r]) (void)hipHostFree(h_idx[r]);
-            if (h_rand[r]) (void)hipHostFree(h_rand[r]);
+            if (h_blk[r]) (void)hipHostFree(h_blk[r]);
             if (ring_ev[r]) (void)hipEventDestroy(ring_ev[r]);
         }
     }
@@ -1401,20 +1397,22 @@ int batch_round(tsg_batch* b) {
         uint32_t cap = std::max<uint32_t>(nu, 2 * b->cap_units);
         HIP_TRY(hipDeviceSynchronize());
         for (int q = 0; q < tsg_batch::kRing; ++q) {
-            if (b->h_units[q]) HIP_TRY(hipHostFree(b->h_units[q]));
-            if (b->h_idx[q]) HIP_TRY(hipHostFree(b->h_idx[q]));
-            if (b->h_rand[q]) HIP_TRY(hipHostFree(b->h_rand[q]));
-            HIP_TRY(hipHostMalloc((void**)&b->h_units[q], cap * sizeof(BatchArgs)));
-            HIP_TRY(hipHostMalloc((void**)&b->h_idx[q], 5 * (cap + 2) * sizeof(uint32_t)));
-            HIP_TRY(hipHostMalloc((void**)&b->h_rand[q], cap * sizeof(RandArgs)));
-            HIP_TRY(b->d_units[q].alloc(cap));
-            HIP_TRY(b->d_idx[q].alloc(5 * (cap + 2)));
-            HIP_TRY(b->d_rand[q].alloc(cap));
+            auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+            b->off_rand = up(5 * (size_t)(cap + 2) * sizeof(uint32_t));
+            b->off_units = up(b->off_rand + cap * sizeof(RandArgs));
+            const size_t bytes = b->off_units + cap * sizeof(BatchArgs);
+            if (b->h_blk[q]) HIP_TRY(hipHostFree(b->h_blk[q]));
+            HIP_TRY(hipHostMalloc((void**)&b->h_blk[q], bytes));
+            HIP_TRY(b->d_blk[q].alloc(bytes));
         }
         b->cap_units = cap;
     }
-    BatchArgs* hu = b->h_units[r];
-    uint32_t* hi = b->h_idx[r];
+    BatchArgs* hu = reinterpret_cast<BatchArgs*>(b->h_blk[r] + b->off_units);
+    uint32_t* hi = reinterpret_cast<uint32_t*>(b->h_blk[r]);
+    RandArgs* hr = reinterpret_cast<RandArgs*>(b->h_blk[r] + b->off_rand);
+    const BatchArgs* d_units = reinterpret_cast<const BatchArgs*>(b->d_blk[r].p + b->off_units);
+    const uint32_t* d_idx = reinterpret_cast<const uint32_t*>(b->d_blk[r].p);
+    const RandArgs* d_rand = reinterpret_cast<const RandArgs*>(b->d_blk[r].p + b->off_rand);
     // layout: kind 0 units then kind 1 units; idx = wg starts (cap+1) then item starts (cap+1), per kind
     uint32_t off = 0, n_wg[2] = {0, 0}, n_items[2] = {0, 0};
     uint32_t* wg_s[2] = {hi, hi + (b->cap_units + 1)};
@@ -1447,21 +1445,20 @@ int batch_round(tsg_batch* b) {
     uint32_t* rs = hi + 4 * (b->cap_units + 1);
     uint32_t n_rand = 0;
     for (size_t q = 0; q < rands.size(); ++q) {
-        b->h_rand[r][q] = rands[q];
+        hr[q] = rands[q];
         rs[q] = n_rand;
         n_rand += rands[q].item1 - rands[q].item0;
     }
-    HIP_TRY(hipMemcpyAsync(b->d_units[r].p, hu, nu * sizeof(BatchArgs), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(b->d_idx[r].p, hi, 5 * (b->cap_units + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(b->d_blk[r].p, b->h_blk[r], b->off_units + nu * sizeof(BatchArgs), hipMemcpyHostToDevice,
+                           s));
     if (!rands.empty()) {
-        HIP_TRY(hipMemcpyAsync(b->d_rand[r].p, b->h_rand[r], rands.size() * sizeof(RandArgs), hipMemcpyHostToDevice, s));
-        HIP_TRY(launch_rand_multi(b->d_rand[r].p, b->d_idx[r].p + 4 * (b->cap_units + 1), (uint32_t)rands.size(),
+        HIP_TRY(launch_rand_multi(d_rand, d_idx + 4 * (b->cap_units + 1), (uint32_t)rands.size(),
                                   n_rand, s));
         b->launches += 1;
     }
-    const BatchArgs* du[2] = {b->d_units[r].p, b->d_units[r].p + units[0].size()};
-    const uint32_t* dw[2] = {b->d_idx[r].p, b->d_idx[r].p + (b->cap_units + 1)};
-    const uint32_t* di[2] = {b->d_idx[r].p + 2 * (b->cap_units + 1), b->d_idx[r].p + 3 * (b->cap_units + 1)};
+    const BatchArgs* du[2] = {d_units, d_units + units[0].size()};
+    const uint32_t* dw[2] = {d_idx, d_idx + (b->cap_units + 1)};
+    const uint32_t* di[2] = {d_idx + 2 * (b->cap_units + 1), d_idx + 3 * (b->cap_units + 1)};
     for (int kd = 0; kd < 2; ++kd) {
         if (units[kd].empty()) continue;
         HIP_TRY(launch_resolve_multi(du[kd], dw[kd], (uint32_t)units[kd].size(), n_wg[kd], kd == 1, s));
@@ -1773,6 +1770,25 @@ int tsg_job_run(tsg_job* job, tsg_progress_fn progress, void* user) {
     HIP_TRY(hipSetDevice(job->ctx->device));
     std::lock_guard<std::mutex> lk(job->ctx->mu);
     return run_job(job, progress, user);
+}
+
+int tsg_job_download_packed(tsg_job* job, uint32_t* state, uint32_t* resolved_flat, float* resolved_score) {
+    if (!job) return fail(TSG_ERR_INVALID_ARGUMENT, "null job");
+    HIP_TRY(hipSetDevice(job->ctx->device));
+    std::lock_guard<std::mutex> lk(job->ctx->mu);
+    hipStream_t s = job->ctx->stream;
+    const size_t N = job->N, R = job->n_res;
+    DBuf<float> rs;
+    if (resolved_score && R) {
+        HIP_TRY(rs.alloc(R));
+        UnpackArgs ua{job->state.p, 0u, nullptr, nullptr, nullptr, job->rlist.p, job->score.p, (uint32_t)R, rs.p};
+        HIP_TRY(launch_unpack(ua, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    if (state) HIP_TRY(hipMemcpy(state, job->state.p, N * sizeof(uint4), hipMemcpyDeviceToHost));
+    if (resolved_flat && R) HIP_TRY(hipMemcpy(resolved_flat, job->rlist.p, R * 4, hipMemcpyDeviceToHost));
+    if (resolved_score && R) HIP_TRY(hipMemcpy(resolved_score, rs.p, R * 4, hipMemcpyDeviceToHost));
+    return TSG_OK;
 }
 
 int tsg_job_download(tsg_job* job, uint8_t* rgba, uint32_t* coord, uint32_t* ids, uint32_t* resolved_flat,

@@ -121,6 +121,7 @@ def lib():
     L.tsg_job_set_tuning.argtypes = [vp, P(Tuning)]
     L.tsg_job_run.argtypes = [vp, PROGRESS_FN, vp]
     L.tsg_job_download.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.tsg_job_download_packed.argtypes = [vp, vp, vp, vp]
     L.tsg_job_stage_stats.argtypes = [vp, vp, vp, vp, u32]
     L.tsg_job_timing.argtypes = [vp, vp, vp]
     L.tsg_job_set_profiling.argtypes = [vp, C.c_int]
@@ -254,15 +255,21 @@ class Example:
 
 
 class GeneratedImage:
-    """GeneratedImage (lib.rs:378-455): image plus coord / id maps."""
+    """GeneratedImage (lib.rs:378-455): image plus coord / id maps.
+
+    Holds the packed per-pixel state as downloaded (four u32 per pixel:
+    rgba, x | y << 16, coord-map id | id-map id << 16, patch id); `image` is
+    a view of it, `coord` (x, y, map) and `ids` (patch, map) are unpacked on
+    first use."""
 
     KERNELS = ("resolve", "commit", "rand", "recolor", "stats", "prefetch")
 
-    def __init__(self, image, coord, ids, resolved_flat, resolved_score, stage_stats, run_ms, n_batches,
+    def __init__(self, state, size, resolved_flat, resolved_score, stage_stats, run_ms, n_batches,
                  original_maps, kernel_stats=None):
-        self.image = image
-        self.coord = coord
-        self.ids = ids
+        self._state = state
+        self._size = size
+        self._coord = None
+        self._ids = None
         self.resolved_flat = resolved_flat
         self.resolved_score = resolved_score
         self.stage_stats = stage_stats
@@ -271,8 +278,34 @@ class GeneratedImage:
         self.original_maps = original_maps
         self.kernel_stats = kernel_stats
 
+    @property
+    def image(self) -> np.ndarray:
+        w, h = self._size
+        return self._state.view(np.uint8).reshape(h, w, 16)[:, :, :4]
+
+    @property
+    def coord(self) -> np.ndarray:
+        if self._coord is None:
+            st = self._state
+            c = np.empty((st.shape[0], 3), np.uint32)
+            c[:, 0] = st[:, 1] & 0xFFFF
+            c[:, 1] = st[:, 1] >> 16
+            c[:, 2] = st[:, 2] & 0xFFFF
+            self._coord = c
+        return self._coord
+
+    @property
+    def ids(self) -> np.ndarray:
+        if self._ids is None:
+            st = self._state
+            d = np.empty((st.shape[0], 2), np.uint32)
+            d[:, 0] = st[:, 3]
+            d[:, 1] = st[:, 2] >> 16
+            self._ids = d
+        return self._ids
+
     def into_image(self) -> np.ndarray:
-        return self.image
+        return np.ascontiguousarray(self.image)
 
     def stage_mean_costs(self) -> np.ndarray:
         sums, cnts, _ = self.stage_stats
@@ -391,12 +424,10 @@ class Session:
         """Downloads the finished job's maps, statistics and timings."""
         L = lib()
         w, h = self._out.width, self._out.height
-        img = np.empty((h, w, 4), np.uint8)  # written whole by the download
-        coord = np.empty((h * w, 3), np.uint32)
-        ids = np.empty((h * w, 2), np.uint32)
+        state = np.empty((h * w, 4), np.uint32)  # written whole by the download
         rf = np.zeros(h * w, np.uint32)
         rs = np.zeros(h * w, np.float32)
-        _check(L.tsg_job_download(self._job, _ptr(img), _ptr(coord), _ptr(ids), _ptr(rf), _ptr(rs)))
+        _check(L.tsg_job_download_packed(self._job, _ptr(state), _ptr(rf), _ptr(rs)))
         n = 16
         sums = np.zeros(n, np.float64)
         cnts = np.zeros(n, np.uint64)
@@ -419,7 +450,7 @@ class Session:
                 clk = np.zeros((h * w, 16), np.uint64)
                 _check(L.tsg_job_phase_clocks(self._job, _ptr(clk), h * w))
                 kst["phase_clocks"] = clk
-        gen = GeneratedImage(img, coord, ids, rf, rs, (sums[:got], cnts[:got], items[:got]), ms.value,
+        gen = GeneratedImage(state, (w, h), rf, rs, (sums[:got], cnts[:got], items[:got]), ms.value,
                              nb.value, self._original_maps, kst)
         gen.device_bytes = self.footprint()  # every buffer the run grew
         return gen
