@@ -98,8 +98,13 @@ def batched_leg(ts, synth, n):
     batch = ts.Batch(ts.Context.default(), max_active=n)
     t0 = time.perf_counter()
     tickets = [batch.submit(s) for s in sess]
-    batch.run_all()
-    res = [batch.take(t) for t in tickets]
+    # results are taken (downloaded) as jobs finish, while later rounds run
+    done = {}
+    while len(done) < n:
+        batch.step(8)
+        for t, status in batch.poll():
+            done[t] = batch.take(t)
+    res = [done[t] for t in tickets]
     t_bat = time.perf_counter() - t0
     got = [digest(g) for g in res]
     del res

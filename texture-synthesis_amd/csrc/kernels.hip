@@ -58,6 +58,7 @@ __device__ __forceinline__ int packed_dy(int32_t o) { return o >> 16; }
 
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr int kTermCap = 512;  // terms per chunk of the pruned evaluation
+constexpr uint32_t kLead = 8;  // coherence candidates evaluated before the rest
 
 struct WaveLds {
     union {
@@ -554,7 +555,26 @@ __device__ __forceinline__ void eval_group(const MapDesc* maps, WaveLds* L, cons
         uint32_t myidx = kNone, xy = 0, ms = 0;
         if (act) {
             s = L->act_sum[lane];
-            for (uint32_t jj = 0; jj < jn; ++jj) s = __fadd_rn(s, L->terms[(lane << sh) + jj]);
+            // the terms are loaded 16 at a time ahead of the in-order adds
+            const float* tp = &L->terms[lane << sh];
+            uint32_t jj = 0;
+            if (sh >= 2) {
+                for (; jj + 16 <= jn; jj += 16) {
+                    const float4 q0 = *reinterpret_cast<const float4*>(tp + jj);
+                    const float4 q1 = *reinterpret_cast<const float4*>(tp + jj + 4);
+                    const float4 q2 = *reinterpret_cast<const float4*>(tp + jj + 8);
+                    const float4 q3 = *reinterpret_cast<const float4*>(tp + jj + 12);
+                    s = __fadd_rn(s, q0.x); s = __fadd_rn(s, q0.y); s = __fadd_rn(s, q0.z); s = __fadd_rn(s, q0.w);
+                    s = __fadd_rn(s, q1.x); s = __fadd_rn(s, q1.y); s = __fadd_rn(s, q1.z); s = __fadd_rn(s, q1.w);
+                    s = __fadd_rn(s, q2.x); s = __fadd_rn(s, q2.y); s = __fadd_rn(s, q2.z); s = __fadd_rn(s, q2.w);
+                    s = __fadd_rn(s, q3.x); s = __fadd_rn(s, q3.y); s = __fadd_rn(s, q3.z); s = __fadd_rn(s, q3.w);
+                }
+                for (; jj + 4 <= jn; jj += 4) {
+                    const float4 q = *reinterpret_cast<const float4*>(tp + jj);
+                    s = __fadd_rn(s, q.x); s = __fadd_rn(s, q.y); s = __fadd_rn(s, q.z); s = __fadd_rn(s, q.w);
+                }
+            }
+            for (; jj < jn; ++jj) s = __fadd_rn(s, tp[jj]);
             myidx = L->act_idx[lane];
             xy = L->act_xy[lane];
             ms = L->act_ms[lane];
@@ -690,9 +710,26 @@ __device__ __forceinline__ uint32_t item_neighbours(const BatchArgs& a, const Ma
     const uint32_t n_coh = (uint32_t)__popcll(cm);
     wave_sync();
     if (lane == 0) {
+        // the reference's sum over the 4k duplicated distances, in order; the
+        // distances are read eight at a time ahead of the adds
         double sum = 0.0;
-        for (uint32_t j = 0; j < kf; ++j) {
-            double d = L->dist[j];
+        uint32_t j = 0;
+        for (; j + 8 <= kf; j += 8) {
+            const double2 d0 = *reinterpret_cast<const double2*>(&L->dist[j]);
+            const double2 d1 = *reinterpret_cast<const double2*>(&L->dist[j + 2]);
+            const double2 d2 = *reinterpret_cast<const double2*>(&L->dist[j + 4]);
+            const double2 d3 = *reinterpret_cast<const double2*>(&L->dist[j + 6]);
+            const double dd[8] = {d0.x, d0.y, d1.x, d1.y, d2.x, d2.y, d3.x, d3.y};
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                sum = sum + dd[q];
+                sum = sum + dd[q];
+                sum = sum + dd[q];
+                sum = sum + dd[q];
+            }
+        }
+        for (; j < kf; ++j) {
+            const double d = L->dist[j];
             sum = sum + d;
             sum = sum + d;
             sum = sum + d;
@@ -921,7 +958,11 @@ __device__ void resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t i
             uint32_t g_lo = 0;
             bool first = true;
             while (g_lo < nv) {
-                const uint32_t g_hi = (first && ncs > 0) ? ncs : min(nv, g_lo + (uint32_t)kWave);
+                // groups: the first kLead coherence candidates (a full evaluation of
+                // a few likely winners gives an early bound), the other coherence
+                // candidates, then the random ones 64 at a time
+                const uint32_t g_hi = g_lo < ncs ? (g_lo == 0 && ncs > 2 * kLead ? kLead : ncs)
+                                                 : min(nv, g_lo + (uint32_t)kWave);
                 // coherence entries of the group
                 if (mine_u) {
                     const uint32_t v = upos - c0;

@@ -1776,18 +1776,20 @@ int tsg_job_download_packed(tsg_job* job, uint32_t* state, uint32_t* resolved_fl
     if (!job) return fail(TSG_ERR_INVALID_ARGUMENT, "null job");
     HIP_TRY(hipSetDevice(job->ctx->device));
     std::lock_guard<std::mutex> lk(job->ctx->mu);
-    hipStream_t s = job->ctx->stream;
+    if (job->step.active) return fail(TSG_ERR_INVALID_ARGUMENT, "the job is still running");
+    // the job's work has completed: the download uses the side stream and so
+    // overlaps whatever else the job stream is running (other jobs' rounds)
+    hipStream_t s = job->ctx->side;
     const size_t N = job->N, R = job->n_res;
-    DBuf<float> rs;
+    float* rs = job->item_score.p;  // per-item scratch, free once the job has run
     if (resolved_score && R) {
-        HIP_TRY(rs.alloc(R));
-        UnpackArgs ua{job->state.p, 0u, nullptr, nullptr, nullptr, job->rlist.p, job->score.p, (uint32_t)R, rs.p};
+        UnpackArgs ua{job->state.p, 0u, nullptr, nullptr, nullptr, job->rlist.p, job->score.p, (uint32_t)R, rs};
         HIP_TRY(launch_unpack(ua, s));
     }
     HIP_TRY(hipStreamSynchronize(s));
     if (state) HIP_TRY(hipMemcpy(state, job->state.p, N * sizeof(uint4), hipMemcpyDeviceToHost));
     if (resolved_flat && R) HIP_TRY(hipMemcpy(resolved_flat, job->rlist.p, R * 4, hipMemcpyDeviceToHost));
-    if (resolved_score && R) HIP_TRY(hipMemcpy(resolved_score, rs.p, R * 4, hipMemcpyDeviceToHost));
+    if (resolved_score && R) HIP_TRY(hipMemcpy(resolved_score, rs, R * 4, hipMemcpyDeviceToHost));
     return TSG_OK;
 }
 
@@ -1796,7 +1798,8 @@ int tsg_job_download(tsg_job* job, uint8_t* rgba, uint32_t* coord, uint32_t* ids
     if (!job) return fail(TSG_ERR_INVALID_ARGUMENT, "null job");
     HIP_TRY(hipSetDevice(job->ctx->device));
     std::lock_guard<std::mutex> lk(job->ctx->mu);
-    hipStream_t s = job->ctx->stream;
+    if (job->step.active) return fail(TSG_ERR_INVALID_ARGUMENT, "the job is still running");
+    hipStream_t s = job->ctx->side;  // the job's work has completed (see tsg_job_download_packed)
     // unpack on the device (8 + 12 + 8 + 4 bytes per pixel), then one copy per map
     const size_t N = job->N, R = job->n_res;
     DBuf<uint32_t> out;
@@ -1870,7 +1873,8 @@ int tsg_job_kernel_stats(tsg_job* job, double* ms, uint64_t* items, uint64_t* la
 void tsg_job_destroy(tsg_job* job) {
     if (!job) return;
     (void)hipSetDevice(job->ctx->device);
-    (void)hipStreamSynchronize(job->ctx->stream);
+    // a finished job has no work in flight (downloads wait for theirs)
+    if (job->step.active) (void)hipStreamSynchronize(job->ctx->stream);
     delete job;
 }
 
