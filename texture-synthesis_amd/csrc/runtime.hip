@@ -351,9 +351,48 @@ struct PicksPool {
     }
 };
 
+// Pinned landing area of downloads: the device writes it at full link
+// speed, then host threads copy into the caller's (pageable) buffers, which
+// spreads the first-touch page faults of fresh buffers over several cores.
+struct Landing {
+    char* buf = nullptr;
+    size_t n = 0;
+    hipError_t reserve(size_t bytes) {
+        if (bytes <= n) return hipSuccess;
+        if (buf) (void)hipHostFree(buf);
+        buf = nullptr;
+        n = 0;
+        hipError_t e = hipHostMalloc((void**)&buf, bytes);
+        if (e == hipSuccess) n = bytes;
+        return e;
+    }
+    ~Landing() {
+        if (buf) (void)hipHostFree(buf);
+    }
+};
+
+// memcpy of n bytes on up to `threads` host threads.
+static void parallel_copy(void* dst, const void* src, size_t n, unsigned threads) {
+    const size_t piece = 1u << 21;
+    if (n <= 2 * piece || threads <= 1) {
+        memcpy(dst, src, n);
+        return;
+    }
+    const unsigned nt = (unsigned)std::min<size_t>(threads, (n + piece - 1) / piece);
+    std::vector<std::thread> th;
+    const size_t per = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        const size_t lo = t * per, hi = std::min(n, lo + per);
+        if (lo >= hi) break;
+        th.emplace_back([=] { memcpy(static_cast<char*>(dst) + lo, static_cast<const char*>(src) + lo, hi - lo); });
+    }
+    for (auto& x : th) x.join();
+}
+
 struct tsg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    Landing landing;
     hipStream_t side = nullptr;  // work that overlaps the stage loop (random candidates ahead)
     Staging staging, staging_side;  // one per stream: a chunk's reuse waits on its stream's event
     PicksPool picks;
@@ -1786,10 +1825,19 @@ int tsg_job_download_packed(tsg_job* job, uint32_t* state, uint32_t* resolved_fl
         UnpackArgs ua{job->state.p, 0u, nullptr, nullptr, nullptr, job->rlist.p, job->score.p, (uint32_t)R, rs};
         HIP_TRY(launch_unpack(ua, s));
     }
+    // device -> pinned landing area -> caller's buffers
+    const size_t b_state = state ? N * sizeof(uint4) : 0, b_flat = resolved_flat ? R * 4 : 0;
+    const size_t b_score = resolved_score ? R * 4 : 0;
+    Landing& L = job->ctx->landing;
+    HIP_TRY(L.reserve(b_state + b_flat + b_score + 1));
+    if (b_state) HIP_TRY(hipMemcpyAsync(L.buf, job->state.p, b_state, hipMemcpyDeviceToHost, s));
+    if (b_flat) HIP_TRY(hipMemcpyAsync(L.buf + b_state, job->rlist.p, b_flat, hipMemcpyDeviceToHost, s));
+    if (b_score) HIP_TRY(hipMemcpyAsync(L.buf + b_state + b_flat, rs, b_score, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    if (state) HIP_TRY(hipMemcpy(state, job->state.p, N * sizeof(uint4), hipMemcpyDeviceToHost));
-    if (resolved_flat && R) HIP_TRY(hipMemcpy(resolved_flat, job->rlist.p, R * 4, hipMemcpyDeviceToHost));
-    if (resolved_score && R) HIP_TRY(hipMemcpy(resolved_score, rs, R * 4, hipMemcpyDeviceToHost));
+    const unsigned nt = std::max(1u, std::min(8u, std::thread::hardware_concurrency() / 2));
+    if (b_state) parallel_copy(state, L.buf, b_state, nt);
+    if (b_flat) parallel_copy(resolved_flat, L.buf + b_state, b_flat, nt);
+    if (b_score) parallel_copy(resolved_score, L.buf + b_state + b_flat, b_score, nt);
     return TSG_OK;
 }
 
