@@ -38,7 +38,8 @@ for p in (ROOT, os.path.join(ROOT, "texture-synthesis_amd")):
 HBM_PEAK_GBS = 8000.0
 K, M = 50, 50
 # algorithmic bytes per pixel-resolve: RGBA8 example gathers of every
-# candidate neighbourhood 4k(k+m), neighbour state 24k, commit 28
+# candidate neighbourhood 4k(k+m), neighbour state 24k (16 B record + 8 B of
+# neighbour list), commit 28
 BYTES_PER_RESOLVE = 4 * K * (K + M) + 24 * K + 28
 WORKLOAD = "plain: 512x512 RGBA8 sinusoid+noise example (seed 512) -> 1024x1024, defaults (k=50, m=50, P=5, p=0.5)"
 
@@ -51,10 +52,11 @@ def build_session(ts, ex, profile=False):
 
 
 def cpu_baseline(ex, threads):
+    import tsgpu as ts
     from oracle import refcpu as R
     job = R.Job(R.default_params(seed=0), [ex], 1024, 1024)
     t0 = time.perf_counter()
-    job.run(threads=threads, px_per_batch=0, redo_shift=5, new_shift=6, max_batch=0)
+    job.run(threads=threads, **ts.default_schedule())
     dt = time.perf_counter() - t0
     return dict(value=round(1024 * 1024 / dt / 1e6, 4), unit="Mpix/s", cores=threads, kind="port",
                 sample=f"one full 1024^2 synthesis of the same workload on {threads} host threads "
@@ -120,6 +122,34 @@ def batched_leg(ts, synth, n):
         "seconds": round(t_bat, 3), "sequential_seconds": round(t_seq, 3),
         "identical_to_alone": seq == got,
         "rounds": st["rounds"], "launches": st["launches"],
+    }
+
+
+def guided_leg(ts, synth, steps):
+    """Multi-example guided synthesis: 4 examples of 300^2 with blob guides,
+    a 1024^2 blob target guide, guide_alpha 0.8, random_init 10, k=50, seed 211."""
+    exs = [synth.example(1000 + i, 300, 300) for i in range(4)]
+    gds = [synth.blobs(2000 + i, 300, 300) for i in range(4)]
+    tg = synth.blobs(3000, 1024, 1024)
+
+    def build():
+        b = ts.Session.builder().seed(211).guide_alpha(0.8).random_init(10).nearest_neighbors(50)
+        for e, g in zip(exs, gds):
+            b.add_example(ts.Example(e).with_guide(g))
+        return b.load_target_guide(tg).output_size(ts.Dims.square(1024)).build()
+
+    build().run()
+    sess = [build() for _ in range(steps)]
+    lat = []
+    for s in sess:
+        t0 = time.perf_counter()
+        s.run()
+        lat.append(time.perf_counter() - t0)
+    return {
+        "workload": "guided: 4 x 300^2 examples with 300^2 blob guides (seeds 1000+i, 2000+i), 1024^2 blob "
+                    "target guide (seed 3000) -> 1024^2, guide_alpha 0.8, random_init 10, k=50, m=50, seed 211",
+        "value": round(1024 * 1024 * steps / sum(lat) / 1e6, 3), "unit": "Mpix/s",
+        "ms_per_step": round(1e3 * sum(lat) / steps, 3), "steps": steps,
     }
 
 
@@ -218,6 +248,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched", type=int, default=256, help="syntheses in the batched leg (0 = skip)")
+    ap.add_argument("--guided", type=int, default=3, help="runs of the guided leg (0 = skip)")
     ap.add_argument("--serve-requests", type=int, default=120, help="requests in the serving leg (0 = skip)")
     ap.add_argument("--serve-qps", type=float, default=12.0)
     ap.add_argument("--serve-budget-gib", type=float, default=4.0)
@@ -250,10 +281,14 @@ def main():
     # profiled run (separate, after the timed region): per-kernel device time
     g = prof.run()
     ks = g.kernel_stats
+    # the dominant kernel: k_resolve, one launch per batch of more than a few
+    # items (the one-workgroup loop over the smallest batches is reported apart);
+    # durations come from the dispatches' own start/end timestamps
     res = ks["resolve"]
-    resolves = res["items"]
+    resolves = res["items"] + ks["resolve_runs"]["items"]
     kern_s = res["ms"] / 1e3
-    achieved = resolves * BYTES_PER_RESOLVE / kern_s / 1e9 if kern_s > 0 else 0.0
+    achieved = res["items"] * BYTES_PER_RESOLVE / kern_s / 1e9 if kern_s > 0 else 0.0
+    per_launch = res["items"] * BYTES_PER_RESOLVE / max(1, res["launches"])
     ms_per_step = t_all * 1e3 / args.steps
     value = 1024 * 1024 * args.steps / t_all / 1e6
     lat_sorted = sorted(lat)
@@ -272,16 +307,19 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": WORKLOAD, "output_px": 1024 * 1024, "resolves_per_step": resolves,
-                   "schedule": "jacobi redo>>5 new>>6", "batches_per_step": g.n_batches},
+                   "schedule": "jacobi " + json.dumps(ts.default_schedule()), "batches_per_step": g.n_batches},
         "latency_ms": {"p50": round(statistics.median(lat), 3), "p99": round(p99, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
                      "kernel": "k_resolve", "bytes_per_resolve": BYTES_PER_RESOLVE,
+                     "algorithmic_bytes_per_launch": round(per_launch),
                      "avg_launch_ms": round(res["ms"] / max(1, res["launches"]), 4),
                      "launches": res["launches"]},
         "kernels_ms": {k: round(v["ms"], 3) for k, v in ks.items()},
         "device_ms_per_step": round(g.run_ms, 3),
     }
+    if args.guided:
+        out["guided"] = guided_leg(ts, synth, args.guided)
     if args.batched:
         out["batched"] = batched_leg(ts, synth, args.batched)
         out["batched"]["vs_plain"] = round(out["batched"]["value"] / value, 2)

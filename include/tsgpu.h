@@ -66,6 +66,7 @@ typedef struct tsg_schedule {
     uint32_t redo_shift;
     uint32_t new_shift;
     uint32_t max_batch;
+    uint32_t first_new_shift;  /* new-pixel shift of the first stage (0: new_shift) */
 } tsg_schedule;
 
 /* Launch structure of the device path. Results never depend on it (every
@@ -177,9 +178,12 @@ int tsg_job_set_profiling(tsg_job* job, int level);
  * unused), [7] kNN search done (in-kernel searches only), [8] ring-search
  * steps, [9] keys buffered before the final selection. */
 int tsg_job_phase_clocks(tsg_job* job, uint64_t* out, uint64_t n_items);
-/* After a profiled run, per kernel [resolve, commit, random candidates,
- * recolour, stats, kNN prefetch]: summed milliseconds, items processed,
- * launches. Each array holds 6 entries. */
+/* After a profiled run, per kernel [resolve (one launch per batch), commit,
+ * random candidates, recolour, stats, kNN prefetch, resolve runs (several
+ * small batches per launch)]: summed milliseconds, items processed,
+ * launches. Each array holds 7 entries. The resolve kernels' milliseconds
+ * are their execution spans on the device clock (earliest workgroup start
+ * to latest workgroup end); the others' come from events around them. */
 int tsg_job_kernel_stats(tsg_job* job, double* ms, uint64_t* items, uint64_t* launches);
 void tsg_job_destroy(tsg_job* job);
 

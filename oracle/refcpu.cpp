@@ -838,7 +838,8 @@ uint64_t batch_end(const orc_job* j, uint64_t i) {
         b = j->sched.px_per_batch;
     } else {
         uint64_t base = j->resolved.size() > j->locked ? j->resolved.size() - j->locked : 0;
-        b = base >> (i < redo_eff ? j->sched.redo_shift : j->sched.new_shift);
+        const uint32_t ns = (j->t == 0 && j->sched.first_new_shift) ? j->sched.first_new_shift : j->sched.new_shift;
+        b = base >> (i < redo_eff ? j->sched.redo_shift : ns);
         if (b < 1) b = 1;
         if (j->sched.max_batch > 0 && b > j->sched.max_batch) b = j->sched.max_batch;
     }

@@ -57,13 +57,15 @@ typedef struct orc_params {
 /* Jacobi batch schedule. px_per_batch > 0 forces fixed-size batches
  * (1 == the reference at max_thread_count(1)). Otherwise a batch inside the
  * redo segment holds max(1, (resolved - locked) >> redo_shift) items and a
- * batch inside the new-pixel segment max(1, (resolved - locked) >> new_shift),
- * both capped at max_batch, never crossing the segment boundary. */
+ * batch inside the new-pixel segment max(1, (resolved - locked) >> new_shift)
+ * (>> first_new_shift in the first stage when that is nonzero), capped at
+ * max_batch, never crossing the segment boundary. */
 typedef struct orc_sched {
     uint32_t px_per_batch;
     uint32_t redo_shift;
     uint32_t new_shift;
     uint32_t max_batch;
+    uint32_t first_new_shift;
 } orc_sched;
 
 typedef struct orc_image { uint32_t w, h; const uint8_t* rgba; } orc_image;

@@ -38,6 +38,7 @@ class Sched(C.Structure):
         ("redo_shift", C.c_uint32),
         ("new_shift", C.c_uint32),
         ("max_batch", C.c_uint32),
+        ("first_new_shift", C.c_uint32),
     ]
 
 
@@ -179,8 +180,8 @@ def default_params(**kw) -> Params:
     return p
 
 
-def sched(px_per_batch=0, redo_shift=1, new_shift=3, max_batch=0) -> Sched:
-    return Sched(px_per_batch, redo_shift, new_shift, max_batch)
+def sched(px_per_batch=0, redo_shift=1, new_shift=3, max_batch=0, first_new_shift=0) -> Sched:
+    return Sched(px_per_batch, redo_shift, new_shift, max_batch, first_new_shift)
 
 
 SEQUENTIAL = dict(px_per_batch=1, redo_shift=0, new_shift=0, max_batch=0)

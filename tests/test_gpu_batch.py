@@ -15,8 +15,8 @@ from tsgpu import synth
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT = dict(px_per_batch=0, redo_shift=5, new_shift=6, max_batch=0)
-SEQ = dict(px_per_batch=1, redo_shift=0, new_shift=0, max_batch=0)
+DEFAULT = ts.default_schedule()
+SEQ = dict(px_per_batch=1, redo_shift=0, new_shift=0, max_batch=0, first_new_shift=0)
 
 
 def spec_list():

@@ -15,7 +15,7 @@ from tsgpu import synth
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT = dict(px_per_batch=0, redo_shift=5, new_shift=6, max_batch=0)
+DEFAULT = ts.default_schedule()
 
 
 def run_pair(examples, out_w, out_h, sched=DEFAULT, threads=8, sample_kind=None, sample_masks=None,

@@ -53,6 +53,7 @@ struct BatchArgs {
     uint64_t* mask_w;
     float* score_w;
     unsigned long long* dbg_clk;  // [stage item][16] phase timestamps (profiling)
+    unsigned long long* dbg_launch;  // {earliest workgroup start, latest end} of this launch (profiling)
     uint32_t team;                // waves per item (1, 2, 4, 8; 0 = 1)
     // neighbour lists computed ahead for stage items [pre_base, pre_end)
     const int32_t* pre_nbrs;      // [item - pre_base][k] packed dx | dy << 16
@@ -145,6 +146,10 @@ hipError_t launch_rank(const uint32_t* rlist, uint32_t lo, uint32_t hi, uint32_t
 // sets the mask bits of rlist[lo, hi); with tile_min, also the least q per tile
 hipError_t launch_mask_or(const uint32_t* rlist, uint32_t lo, uint32_t hi, uint32_t W, uint32_t tiles_x,
                           uint64_t* mask, uint32_t* tile_min, hipStream_t s);
+
+// Events the next resolve launch (launch_resolve / _small / _coop) records
+// around its dispatch, for per-kernel profiling; null for none.
+void set_launch_events(hipEvent_t start, hipEvent_t stop);
 
 hipError_t configure_kernels();
 size_t resolve_lds_bytes(bool guided);

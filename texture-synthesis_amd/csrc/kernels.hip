@@ -15,6 +15,7 @@
 // a separate multiply and add. Build with -ffp-contract=off.
 #include <float.h>
 #include <hip/hip_cooperative_groups.h>
+#include <hip/hip_ext.h>
 #include <math.h>
 
 #include "kernels.h"
@@ -1072,11 +1073,23 @@ __device__ __forceinline__ WaveLds* setup_lds(const BatchArgs& a, unsigned char*
     return reinterpret_cast<WaveLds*>(smem + (GUIDED ? 2 : 1) * 256 * kLutCopies * sizeof(float));
 }
 
-// One batch, one wavefront per item. With a.fused_commit the workgroup that
+// Profiling: the launch's execution span on the device clock, as the
+// earliest workgroup start and the latest workgroup end.
+__device__ __forceinline__ void launch_span(const BatchArgs& a, uint64_t t_start) {
+    if (!a.dbg_launch) return;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicMin(&a.dbg_launch[0], (unsigned long long)t_start);
+        atomicMax(&a.dbg_launch[1], (unsigned long long)wall_clock64());
+    }
+}
+
+// One batch, teams of waves per item. With a.fused_commit the workgroup that
 // finishes last commits the whole batch (the others have written their
-// results and exited), saving the commit launch on small batches.
+// results), saving the commit launch.
 template <bool GUIDED>
 __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve(BatchArgs a) {
+    const uint64_t t_start = a.dbg_launch ? wall_clock64() : 0;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ uint32_t is_last;
     __shared__ MapDesc smaps[kMapCache];
@@ -1089,16 +1102,22 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve(BatchArgs a) {
     const int team = wave / T, member = wave % T;
     const uint32_t item = a.item0 + blockIdx.x * (uint32_t)(kWavesPerWG / T) + (uint32_t)team;
     resolve_team<GUIDED>(a, maps, item, a.item0, a.n_res, wls + team * T, member, T, item < a.item1, lut, glut, lane);
-    if (!a.fused_commit) return;
+    if (!a.fused_commit) {
+        launch_span(a, t_start);
+        return;
+    }
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) is_last = atomicAdd(a.done_counter, 1u) == gridDim.x - 1;
     __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    for (uint32_t i = a.item0 + threadIdx.x; i < a.item1; i += blockDim.x)
-        commit_one(a.W, a.tiles_x, maps, a.work, a.result, i, a.item0, a.first_new, a.state_w, a.mask_w, a.score_w);
-    if (threadIdx.x == 0) *a.done_counter = 0u;
+    if (is_last) {
+        __threadfence();
+        for (uint32_t i = a.item0 + threadIdx.x; i < a.item1; i += blockDim.x)
+            commit_one(a.W, a.tiles_x, maps, a.work, a.result, i, a.item0, a.first_new, a.state_w, a.mask_w,
+                       a.score_w);
+        if (threadIdx.x == 0) *a.done_counter = 0u;
+    }
+    launch_span(a, t_start);
 }
 
 // A run of small consecutive batches (at most kSmallWaves items each) in one
@@ -1107,6 +1126,7 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve(BatchArgs a) {
 template <bool GUIDED>
 __global__ __launch_bounds__(kWave* kSmallWaves) void k_resolve_small(BatchArgs a, const uint32_t* bounds,
                                                                       const uint32_t* nres, uint32_t n_batches) {
+    const uint64_t t_start = a.dbg_launch ? wall_clock64() : 0;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ MapDesc smaps[kMapCache];
     float *lut, *glut;
@@ -1128,6 +1148,7 @@ __global__ __launch_bounds__(kWave* kSmallWaves) void k_resolve_small(BatchArgs 
                                 a.score_w);
         __syncthreads();
     }
+    launch_span(a, t_start);
 }
 
 // A run of consecutive latency-bound batches in one cooperative launch (all
@@ -1391,6 +1412,16 @@ __global__ void k_stats(StatsArgs a) {
 
 }  // namespace
 
+namespace {
+// events the next resolve launch records around its dispatch (profiling)
+hipEvent_t g_ev_start = nullptr, g_ev_stop = nullptr;
+}  // namespace
+
+void set_launch_events(hipEvent_t start, hipEvent_t stop) {
+    g_ev_start = start;
+    g_ev_stop = stop;
+}
+
 hipError_t configure_kernels() {
     hipError_t e = hipFuncSetAttribute((const void*)k_resolve<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)resolve_lds_bytes(true));
@@ -1514,20 +1545,25 @@ hipError_t launch_resolve_coop(const BatchArgs& a, const uint32_t* bounds, const
     BatchArgs args = a;
     void* params[] = {&args, &bounds, &nres, &n_batches, &team_max, &overflow};
     const void* f = guided ? (const void*)k_resolve_coop<true> : (const void*)k_resolve_coop<false>;
-    return hipLaunchCooperativeKernel(f, dim3(wgs), dim3(kWave * kSmallWaves), params,
-                                      (unsigned)resolve_small_lds_bytes(guided), s);
+    hipError_t e;
+    if (g_ev_start && (e = hipEventRecord(g_ev_start, s)) != hipSuccess) return e;
+    if ((e = hipLaunchCooperativeKernel(f, dim3(wgs), dim3(kWave * kSmallWaves), params,
+                                        (unsigned)resolve_small_lds_bytes(guided), s)) != hipSuccess)
+        return e;
+    if (g_ev_stop && (e = hipEventRecord(g_ev_stop, s)) != hipSuccess) return e;
+    return hipSuccess;
 }
 
 hipError_t launch_resolve_small(const BatchArgs& a, const uint32_t* bounds, const uint32_t* nres,
                                 uint32_t n_batches, bool guided, hipStream_t s) {
     if (n_batches == 0) return hipSuccess;
-    size_t lds = resolve_small_lds_bytes(guided);
+    const uint32_t lds = (uint32_t)resolve_small_lds_bytes(guided);
     if (guided) {
-        hipLaunchKernelGGL(k_resolve_small<true>, dim3(1), dim3(kWave * kSmallWaves), lds, s, a, bounds, nres,
-                           n_batches);
+        hipExtLaunchKernelGGL(k_resolve_small<true>, dim3(1), dim3(kWave * kSmallWaves), lds, s, g_ev_start, g_ev_stop,
+                              0u, a, bounds, nres, n_batches);
     } else {
-        hipLaunchKernelGGL(k_resolve_small<false>, dim3(1), dim3(kWave * kSmallWaves), lds, s, a, bounds, nres,
-                           n_batches);
+        hipExtLaunchKernelGGL(k_resolve_small<false>, dim3(1), dim3(kWave * kSmallWaves), lds, s, g_ev_start,
+                              g_ev_stop, 0u, a, bounds, nres, n_batches);
     }
     return hipGetLastError();
 }
@@ -1538,11 +1574,11 @@ hipError_t launch_resolve(const BatchArgs& a, bool guided, hipStream_t s) {
     if (a.team > kWavesPerWG || (a.team & (a.team - 1)) != 0) return hipErrorInvalidValue;
     const uint32_t ipw = items_per_wg(a.team);
     dim3 grid((n + ipw - 1) / ipw), block(kWave * kWavesPerWG);
-    size_t lds = resolve_lds_bytes(guided);
+    const uint32_t lds = (uint32_t)resolve_lds_bytes(guided);
     if (guided) {
-        hipLaunchKernelGGL(k_resolve<true>, grid, block, lds, s, a);
+        hipExtLaunchKernelGGL(k_resolve<true>, grid, block, lds, s, g_ev_start, g_ev_stop, 0u, a);
     } else {
-        hipLaunchKernelGGL(k_resolve<false>, grid, block, lds, s, a);
+        hipExtLaunchKernelGGL(k_resolve<false>, grid, block, lds, s, g_ev_start, g_ev_stop, 0u, a);
     }
     return hipGetLastError();
 }

@@ -67,6 +67,7 @@ class Schedule(C.Structure):
         ("redo_shift", C.c_uint32),
         ("new_shift", C.c_uint32),
         ("max_batch", C.c_uint32),
+        ("first_new_shift", C.c_uint32),
     ]
 
 
@@ -81,6 +82,13 @@ class Tuning(C.Structure):
         ("coop_wgs", C.c_uint32),
         ("coop_max", C.c_uint32),
     ]
+
+
+def default_schedule() -> dict:
+    """The library's default Jacobi schedule as keyword arguments."""
+    s = Schedule()
+    lib().tsg_schedule_default(C.byref(s))
+    return {n: getattr(s, n) for n, _ in Schedule._fields_}
 
 
 def default_tuning() -> dict:
@@ -262,7 +270,7 @@ class GeneratedImage:
     a view of it, `coord` (x, y, map) and `ids` (patch, map) are unpacked on
     first use."""
 
-    KERNELS = ("resolve", "commit", "rand", "recolor", "stats", "prefetch")
+    KERNELS = ("resolve", "commit", "rand", "recolor", "stats", "prefetch", "resolve_runs")
 
     def __init__(self, state, size, resolved_flat, resolved_score, stage_stats, run_ms, n_batches,
                  original_maps, kernel_stats=None):
@@ -376,8 +384,9 @@ class Session:
     def builder() -> "SessionBuilder":
         return SessionBuilder()
 
-    def set_schedule(self, px_per_batch=0, redo_shift=5, new_shift=6, max_batch=0) -> "Session":
-        s = Schedule(px_per_batch, redo_shift, new_shift, max_batch)
+    def set_schedule(self, px_per_batch=0, redo_shift=5, new_shift=6, max_batch=0, first_new_shift=7) -> "Session":
+        """Jacobi batch schedule (defaults: the library's, see default_schedule())."""
+        s = Schedule(px_per_batch, redo_shift, new_shift, max_batch, first_new_shift)
         _check(lib().tsg_job_set_schedule(self._job, C.byref(s)))
         return self
 
@@ -440,9 +449,9 @@ class Session:
         _check(L.tsg_job_timing(self._job, C.byref(ms), C.byref(nb)))
         kst = None
         if getattr(self, "_profiled", False):
-            kms = np.zeros(6, np.float64)
-            kit = np.zeros(6, np.uint64)
-            kla = np.zeros(6, np.uint64)
+            kms = np.zeros(7, np.float64)
+            kit = np.zeros(7, np.uint64)
+            kla = np.zeros(7, np.uint64)
             _check(L.tsg_job_kernel_stats(self._job, _ptr(kms), _ptr(kit), _ptr(kla)))
             kst = {n: dict(ms=float(kms[i]), items=int(kit[i]), launches=int(kla[i]))
                    for i, n in enumerate(GeneratedImage.KERNELS)}
