@@ -89,7 +89,14 @@ typedef struct tsg_schedule {
  *                runs consecutive latency-bound batches with grid barriers
  *                instead of two launches per batch (0 = off)
  *   coop_max     batches of at most this many items (<= 16 * coop_wgs) go
- *                to that launch */
+ *                to that launch
+ *   flow         1: with `ahead`, each stage's leading run of latency-bound
+ *                new-pixel batches goes to one launch of persistent
+ *                workgroups that resolve the items in order, each waiting
+ *                only for the neighbours committed inside the run (per-pixel
+ *                flags); 0: one launch (or loop step) per batch
+ *   flow_wgs     workgroups of that launch (0: as many as are resident)
+ *   flow_team    waves per item in that launch (1, 2, 4 or 8) */
 typedef struct tsg_tuning {
     uint32_t small_max;
     uint32_t fuse_max;
@@ -99,6 +106,9 @@ typedef struct tsg_tuning {
     uint32_t ahead;
     uint32_t coop_wgs;
     uint32_t coop_max;
+    uint32_t flow;
+    uint32_t flow_wgs;
+    uint32_t flow_team;
 } tsg_tuning;
 
 typedef struct tsg_image {

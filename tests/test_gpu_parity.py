@@ -237,7 +237,11 @@ def test_default_schedule_bit_exact_c1(ctx, seed):
          coop_wgs=0, coop_max=0),
     dict(small_max=16, fuse_max=2048, team_max=4, ahead=1, coop_wgs=0, coop_max=0),
     dict(coop_wgs=8, coop_max=128, ahead=0),
-    dict(coop_wgs=256, coop_max=4096, team_max=8),
+    dict(coop_wgs=256, coop_max=4096, team_max=8, flow=0),
+    dict(flow=0),
+    dict(flow=1, flow_team=1, flow_wgs=3),
+    dict(flow=1, flow_team=2, flow_wgs=1),
+    dict(flow=1, flow_team=4, prefetch_max=64),
 ])
 def test_launch_tuning_does_not_change_results(ctx, tuning):
     # every launch structure resolves the same batches against the same states

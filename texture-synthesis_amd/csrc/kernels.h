@@ -81,6 +81,29 @@ struct PrefetchArgs {
     uint32_t item0, item1;
     int32_t* out_nbrs;            // [item - item0][k]
     uint32_t* out_cnt;
+    // optional: per item, bit j set when neighbour j is a pixel of mask_new
+    uint64_t* out_dep;            // [item - item0]
+};
+
+// A run of consecutive new-pixel items [a.item0, a.item0 + n) of one stage in
+// one launch of persistent workgroups. Every item has its neighbour list
+// (a.pre_nbrs) and a dependency mask dep[item - a.pre_base]: bit j set when
+// neighbour j is a pixel committed by an earlier item of the run. Workgroups
+// take items in order from a ticket counter, wait until each dependency's
+// flag holds `epoch`, resolve, commit their item's pixel and publish it by
+// writing `epoch` into its flag. Items wait only on earlier items, which
+// running workgroups hold, so the run always drains. A wait longer than
+// spin_limit polls sets bit 2 of *overflow and ends every wait of the launch
+// (the results are then invalid and the host reports the failure).
+struct FlowArgs {
+    uint32_t* ticket;
+    uint32_t ticket_base;         // counter value at the launch start
+    uint32_t n;
+    const uint64_t* dep;
+    uint32_t* flag;               // per pixel: epoch of the run that committed it
+    uint32_t epoch;
+    uint32_t spin_limit;
+    uint32_t* overflow;
 };
 
 struct RandArgs {
@@ -174,6 +197,11 @@ hipError_t launch_resolve_coop(const BatchArgs& a, const uint32_t* bounds, const
 hipError_t launch_resolve_small(const BatchArgs& a, const uint32_t* bounds, const uint32_t* nres,
                                 uint32_t n_batches, bool guided, hipStream_t s);
 hipError_t launch_resolve(const BatchArgs& a, bool guided, hipStream_t s);
+// A dependency-ordered run (FlowArgs): wgs persistent workgroups of `team`
+// waves; wgs <= flow_max_wgs(team, guided). Consumes n + wgs tickets.
+uint32_t flow_max_wgs(uint32_t team, bool guided);
+hipError_t launch_resolve_flow(const BatchArgs& a, const FlowArgs& f, uint32_t wgs, uint32_t team, bool guided,
+                               hipStream_t s);
 hipError_t launch_stats(const StatsArgs& a, hipStream_t s);
 hipError_t launch_rand(const RandArgs& a, hipStream_t s);
 hipError_t launch_commit(const CommitArgs& a, hipStream_t s);

@@ -877,10 +877,42 @@ __device__ __forceinline__ void item_winner(const BatchArgs& a, const MapDesc* m
 // frozen at the batch start (n_res resolved pixels); writes result[item -
 // item0]. Lt: the team's WaveLds (member q uses Lt + q); `valid` is false
 // for members of a team past the batch end, which still take the barriers.
+// Flow runs, leader: waits until every neighbour committed inside the run
+// (dependency bit set) has published its flag, then acquires at device scope
+// so that the team's state reads see those commits.
+__device__ __forceinline__ void flow_wait(const FlowArgs& f, const BatchArgs& a, uint32_t item, int x, int y,
+                                          uint32_t kf, const WaveLds* L, int lane) {
+    const uint64_t dep = f.dep[item - a.pre_base] & (kf >= 64u ? ~0ull : ((1ull << kf) - 1ull));
+    if (dep == 0) return;
+    uint32_t q = 0;
+    bool need = (dep >> lane) & 1ull;
+    if (need) {
+        const int32_t o = L->nd[lane];
+        q = (uint32_t)(y + packed_dy(o)) * a.W + (uint32_t)(x + packed_dx(o));
+    }
+    uint32_t polls = 0;
+    for (;;) {
+        if (need) need = __hip_atomic_load(f.flag + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != f.epoch;
+        if (__ballot(need) == 0ull) break;
+        ++polls;
+        if ((polls & 63u) == 0u) {
+            // another wave gave up: end this wait too
+            const uint32_t ov = __hip_atomic_load(f.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (ov & 4u) break;
+        }
+        if (polls >= f.spin_limit) {
+            if (lane == 0) atomicOr(f.overflow, 4u);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
 template <bool GUIDED>
 __device__ void resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t item, uint32_t item0, uint32_t n_res,
                              WaveLds* Lt, int member, int T, bool valid, const float* lut, const float* glut,
-                             int lane) {
+                             int lane, const FlowArgs* flow = nullptr) {
     WaveLds* L = Lt + member;
     const int slot = lane % kLutCopies;
     const uint32_t rel = item - item0;
@@ -915,6 +947,7 @@ __device__ void resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t i
     if (valid && lead) {
         const uint32_t kf0 = item_knn(a, item, rel, x, y, n_res, L, lane);
         if (lane == 0) L->t_kf = kf0;
+        if (flow) flow_wait(*flow, a, item, x, y, kf0, L, lane);
     }
     if (T > 1) __syncthreads();
     else wave_sync();
@@ -1190,6 +1223,39 @@ __global__ __launch_bounds__(kWave* kSmallWaves) void k_resolve_coop(BatchArgs a
     }
 }
 
+// A dependency-ordered run of new-pixel items (FlowArgs): one item per
+// workgroup at a time, the workgroup's waves as its team. Thread 0 is the
+// team leader's lane 0: it writes the item's result into the workgroup's
+// slot, commits the pixel and publishes it with a device-scope release.
+template <bool GUIDED>
+__global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve_flow(BatchArgs a, FlowArgs f) {
+    const uint64_t t_start = a.dbg_launch ? wall_clock64() : 0;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ MapDesc smaps[kMapCache];
+    __shared__ uint32_t s_idx;
+    float *lut, *glut;
+    const MapDesc* maps;
+    WaveLds* wls = setup_lds<GUIDED>(a, smem, lut, glut, smaps, maps);
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int T = (int)(blockDim.x / kWave);
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_idx = atomicAdd(f.ticket, 1u) - f.ticket_base;
+        __syncthreads();
+        const uint32_t idx = s_idx;
+        if (idx >= f.n) break;
+        const uint32_t item = a.item0 + idx;
+        const uint32_t slot0 = item - blockIdx.x;  // result[item - slot0] is this workgroup's slot
+        resolve_team<GUIDED>(a, maps, item, slot0, 0u, wls, wave, T, true, lut, glut, lane, &f);
+        if (threadIdx.x == 0) {
+            commit_one(a.W, a.tiles_x, maps, a.work, a.result, item, slot0, 0u, a.state_w, a.mask_w, a.score_w);
+            __hip_atomic_store(f.flag + a.work[item], f.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    launch_span(a, t_start);
+}
+
 // Index u with start[u] <= g < start[u + 1] (start[0] = 0, n entries).
 __device__ __forceinline__ uint32_t find_unit(const uint32_t* start, uint32_t n, uint32_t g) {
     uint32_t lo = 0, hi = n - 1;
@@ -1251,6 +1317,15 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_knn_prefetch(PrefetchArg
     const uint32_t kf = knn_search(v, L, (int)(flat % a.W), (int)(flat / a.W), a.k, n_at, lane, iters, buffered);
     if (lane < (int)kf) a.out_nbrs[(size_t)rel * a.k + lane] = L->nd[lane];
     if (lane == 0) a.out_cnt[rel] = kf;
+    if (a.out_dep) {
+        bool d = false;
+        if (lane < (int)kf && a.mask_new) {
+            const int32_t o = L->nd[lane];
+            d = px_set(a.mask_new, a.tiles_x, (int)(flat % a.W) + packed_dx(o), (int)(flat / a.W) + packed_dy(o));
+        }
+        const uint64_t dm = __ballot(d);
+        if (lane == 0) a.out_dep[rel] = dm;
+    }
 }
 
 __global__ void k_unpack(UnpackArgs a) {
@@ -1444,6 +1519,12 @@ hipError_t configure_kernels() {
     e = hipFuncSetAttribute((const void*)k_resolve_coop<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)resolve_small_lds_bytes(false));
     if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)k_resolve_flow<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)resolve_lds_bytes(true));
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)k_resolve_flow<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)resolve_lds_bytes(false));
+    if (e != hipSuccess) return e;
     return hipFuncSetAttribute((const void*)k_resolve_small<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)resolve_small_lds_bytes(false));
 }
@@ -1564,6 +1645,43 @@ hipError_t launch_resolve_small(const BatchArgs& a, const uint32_t* bounds, cons
     } else {
         hipExtLaunchKernelGGL(k_resolve_small<false>, dim3(1), dim3(kWave * kSmallWaves), lds, s, g_ev_start,
                               g_ev_stop, 0u, a, bounds, nres, n_batches);
+    }
+    return hipGetLastError();
+}
+
+size_t flow_lds_bytes(uint32_t team, bool guided) {
+    return (guided ? 2 : 1) * 256 * kLutCopies * sizeof(float) + team * sizeof(WaveLds);
+}
+
+uint32_t flow_max_wgs(uint32_t team, bool guided) {
+    if (team == 0 || team > (uint32_t)kWavesPerWG) return 0;
+    int per_cu = 0, cus = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    const void* f = guided ? (const void*)k_resolve_flow<true> : (const void*)k_resolve_flow<false>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kWave * team, flow_lds_bytes(team, guided)) !=
+        hipSuccess)
+        return 0;
+    return (uint32_t)(per_cu * cus);
+}
+
+hipError_t launch_resolve_flow(const BatchArgs& a, const FlowArgs& f, uint32_t wgs, uint32_t team, bool guided,
+                               hipStream_t s) {
+    if (f.n == 0) return hipSuccess;
+    if (team == 0 || team > (uint32_t)kWavesPerWG || (team & (team - 1)) != 0) return hipErrorInvalidValue;
+    if (wgs == 0 || !f.ticket || !f.dep || !f.flag || !f.overflow || f.epoch == 0 || f.spin_limit == 0)
+        return hipErrorInvalidValue;
+    // every item needs its neighbour list; test hooks are not supported
+    if (!a.pre_nbrs || a.item0 < a.pre_base || a.item0 + f.n > a.pre_end || a.ext_nbrs || a.ext_cands ||
+        a.dbg_nbrs || a.dbg_best || a.knn_only || a.ext_full_cost)
+        return hipErrorInvalidValue;
+    const uint32_t lds = (uint32_t)flow_lds_bytes(team, guided);
+    if (guided) {
+        hipExtLaunchKernelGGL(k_resolve_flow<true>, dim3(wgs), dim3(kWave * team), lds, s, g_ev_start, g_ev_stop, 0u,
+                              a, f);
+    } else {
+        hipExtLaunchKernelGGL(k_resolve_flow<false>, dim3(wgs), dim3(kWave * team), lds, s, g_ev_start, g_ev_stop,
+                              0u, a, f);
     }
     return hipGetLastError();
 }

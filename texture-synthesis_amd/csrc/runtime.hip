@@ -446,6 +446,7 @@ struct tsg_job {
         DBuf<uint32_t> cnt, n_at;
         DBuf<uint64_t> mend, mnew;  // resolved mask at the stage end; the stage's picks
         DBuf<uint32_t> tmin;        // per tile: least position of its picks
+        DBuf<uint64_t> dep;         // per prefetched item: neighbours that are the stage's picks
         hipEvent_t ready = nullptr, done = nullptr;
         bool done_recorded = false;
         int stage = -1;
@@ -469,6 +470,11 @@ struct tsg_job {
     DBuf<uint32_t> pre_n;      // per window item: resolved count at its batch start
     DBuf<int32_t> pre_nbrs;    // per window item: k packed offsets
     DBuf<uint32_t> pre_cnt;
+    // dependency-ordered runs: per pixel the epoch of the run that committed
+    // it, the runs' ticket counter and its value as the next run starts
+    DBuf<uint32_t> flow_flag;
+    DBuf<uint32_t> flow_ticket;
+    uint32_t flow_epoch = 0, flow_tickets = 0;
     size_t small_used = 0;
     // host bookkeeping
     std::vector<uint32_t> unresolved;
@@ -681,6 +687,7 @@ enum { K_RESOLVE = 0, K_COMMIT = 1, K_RAND = 2, K_RECOLOR = 3, K_STATS = 4, K_PR
 constexpr int kKinds = 7;
 constexpr size_t kEngineRcBudget = 4u << 20;  // random-candidate chunk of a job under the engine
 constexpr uint64_t kPrefetchCap = 65536;       // items per prefetched kNN window
+constexpr uint32_t kFlowSpin = 1u << 18;       // polls of one dependency wait before a run gives up
 
 hipError_t next_event(tsg_job* j, hipEvent_t* e) {
     if (j->ev_next == j->ev_pool.size()) {
@@ -782,7 +789,7 @@ int wait_picks(tsg_job* j, int t) {
 // Device bytes of the stage-ahead buffers (two sets sized for the largest stage).
 uint64_t ahead_bytes(const tsg_job* j) {
     const uint64_t S = std::max<uint64_t>(1, stage_size(j, 0));
-    const uint64_t per = S * (j->prm.random_sample_locations * sizeof(uint2) + j->prm.nearest_neighbors * 4 + 8) +
+    const uint64_t per = S * (j->prm.random_sample_locations * sizeof(uint2) + j->prm.nearest_neighbors * 4 + 16) +
                          (uint64_t)j->tiles_x * j->tiles_y * 20;
     return 2 * per;
 }
@@ -864,6 +871,7 @@ int ahead_stage(tsg_job* j) {
         pa.item1 = (uint32_t)w_end;
         pa.out_nbrs = A.nbrs.p;
         pa.out_cnt = A.cnt.p;
+        pa.out_dep = A.dep.p;
         HIP_TRY(timed_launch_on(j, side, K_PREFETCH, w_end, [&] { return launch_knn_prefetch(pa, side); }));
     }
     HIP_TRY(hipEventRecord(A.ready, side));
@@ -1209,12 +1217,15 @@ int step_finish(tsg_job* j) {
     HIP_TRY(hipMemcpy(&ovf, j->overflow.p, 4, hipMemcpyDeviceToHost));
     j->step.active = false;
     j->ran = true;
+    if (ovf & 4u) return fail(TSG_ERR_DEVICE, "a dependency wait of a flow run timed out");
     if (ovf & 2u) return fail(TSG_ERR_DEVICE, "a batch exceeded its cooperative launch");
     if (ovf) return fail(TSG_ERR_SAMPLING, "a random-candidate rejection loop hit its bound (sample mask too sparse)");
     return TSG_OK;
 }
 
-// Single job: runs of small batches go to one single-workgroup launch; other
+// Single job: with stage-ahead neighbour lists, each stage's leading run of
+// new-pixel batches goes to one dependency-ordered launch (k_resolve_flow);
+// other runs of small batches go to one single-workgroup launch; other
 // batches get one k_resolve launch, committing in its last workgroup when
 // the batch holds at most fuse_max items.
 int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
@@ -1234,6 +1245,7 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
             HIP_TRY(A.mend.reserve((size_t)j->tiles_x * j->tiles_y));
             HIP_TRY(A.mnew.reserve((size_t)j->tiles_x * j->tiles_y));
             HIP_TRY(A.tmin.reserve((size_t)j->tiles_x * j->tiles_y));
+            HIP_TRY(A.dep.reserve(smax));
             if (!A.ready) HIP_TRY(hipEventCreateWithFlags(&A.ready, hipEventDisableTiming));
         }
         j->ahead_open = j->n_res;
@@ -1245,8 +1257,43 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
         HIP_TRY(hipStreamWaitEvent(j->ctx->side, e, 0));
         if ((rc = ahead_stage(j))) return rc;
     }
+    // dependency-ordered runs need the stage-ahead neighbour lists
+    bool flow_on = j->ahead_on && j->tune.flow;
+    const uint32_t flow_team = j->tune.flow_team;
+    uint32_t flow_wgs = 0;
+    if (flow_on) {
+        const uint32_t mx = flow_max_wgs(flow_team, j->guided);
+        flow_wgs = j->tune.flow_wgs ? std::min(j->tune.flow_wgs, mx) : mx;
+        flow_on = flow_wgs > 0;
+    }
+    if (flow_on) {
+        if (j->flow_flag.n < j->N) {
+            HIP_TRY(j->flow_flag.alloc(j->N));
+            HIP_TRY(hipMemsetAsync(j->flow_flag.p, 0, j->N * 4, s));
+            j->flow_epoch = 0;
+        }
+        if (!j->flow_ticket.n) {
+            HIP_TRY(j->flow_ticket.alloc(1));
+            HIP_TRY(hipMemsetAsync(j->flow_ticket.p, 0, 4, s));
+            j->flow_tickets = 0;
+        }
+        HIP_TRY(j->result.reserve(flow_wgs));
+    }
     int last_pct = 0;
     std::vector<uint8_t> snap;
+    auto progress = [&](uint64_t i1) -> int {
+        int pct = (int)std::lround((double)(j->overall_cur + i1) / (double)std::max<uint64_t>(j->overall_total, 1) *
+                                   100.0);
+        if (pct == last_pct) return TSG_OK;
+        last_pct = pct;
+        snap.resize(j->N * 4);
+        HIP_TRY(hipStreamSynchronize(s));
+        std::vector<uint4> stv(j->N);
+        HIP_TRY(hipMemcpy(stv.data(), j->state.p, j->N * sizeof(uint4), hipMemcpyDeviceToHost));
+        for (size_t q = 0; q < j->N; ++q) memcpy(&snap[q * 4], &stv[q].x, 4);
+        cb(user, snap.data(), j->W, j->H, j->overall_cur + i1, j->overall_total, i1, j->step.S);
+        return TSG_OK;
+    };
     std::vector<uint32_t> sb_bounds, sb_nres;
     bool sb_coop = false;  // the pending run goes to the cooperative kernel
     const uint32_t coop_wgs = std::min(j->tune.coop_wgs, j->tune.coop_wgs ? coop_max_wgs(j->guided) : 0u);
@@ -1283,6 +1330,34 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
         if ((rc = step_open_stage(j))) return rc;
         uint64_t i0, i1;
         while (step_peek(j, i0, i1)) {
+            tsg_job::Step& st = j->step;
+            const uint64_t flow_end = flow_on ? std::min<uint64_t>(j->ahead[st.t % 2].w_end, st.S_eff) : 0;
+            if (flow_on && i0 == st.redo_eff && flow_end > i0) {
+                // the stage's new-pixel batches with neighbour lists: one
+                // dependency-ordered run
+                if ((rc = flush_small())) return rc;
+                BatchArgs& a = st.a;
+                a.item0 = (uint32_t)i0;
+                a.item1 = (uint32_t)flow_end;
+                a.n_res = (uint32_t)j->n_res;
+                if (++j->flow_epoch == 0) {
+                    HIP_TRY(hipMemsetAsync(j->flow_flag.p, 0, j->N * 4, s));
+                    j->flow_epoch = 1;
+                }
+                const uint32_t n = (uint32_t)(flow_end - i0);
+                const uint32_t g = std::min(flow_wgs, n);
+                const FlowArgs f{j->flow_ticket.p, j->flow_tickets, n, j->ahead[st.t % 2].dep.p,
+                                 j->flow_flag.p, j->flow_epoch, kFlowSpin, j->overflow.p};
+                HIP_TRY(timed_launch(j, K_RUN, n, [&] { return launch_resolve_flow(a, f, g, flow_team, j->guided, s); }));
+                j->flow_tickets += n + g;
+                while (st.i < flow_end) {
+                    const uint64_t b0 = st.i, b1 = batch_end(j, b0, st.redo, st.S_eff);
+                    step_advance(j, b0, b1);
+                }
+                if (st.i != flow_end) return fail(TSG_ERR_DEVICE, "dependency-ordered run out of step");
+                if (cb && (rc = progress(flow_end))) return rc;
+                continue;
+            }
             if (step_wants_prefetch(j, i0, i1)) {
                 // earlier small batches read the previous window: launch them first
                 if ((rc = flush_small())) return rc;
@@ -1314,17 +1389,7 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
             step_advance(j, i0, i1);
             if (cb) {
                 if ((rc = flush_small())) return rc;
-                int pct = (int)std::lround((double)(j->overall_cur + i1) /
-                                           (double)std::max<uint64_t>(j->overall_total, 1) * 100.0);
-                if (pct != last_pct) {
-                    last_pct = pct;
-                    snap.resize(j->N * 4);
-                    HIP_TRY(hipStreamSynchronize(s));
-                    std::vector<uint4> stv(j->N);
-                    HIP_TRY(hipMemcpy(stv.data(), j->state.p, j->N * sizeof(uint4), hipMemcpyDeviceToHost));
-                    for (size_t q = 0; q < j->N; ++q) memcpy(&snap[q * 4], &stv[q].x, 4);
-                    cb(user, snap.data(), j->W, j->H, j->overall_cur + i1, j->overall_total, i1, j->step.S);
-                }
+                if ((rc = progress(i1))) return rc;
             }
         }
         if ((rc = flush_small())) return rc;
@@ -1347,7 +1412,8 @@ size_t job_footprint(const tsg_job* j) {
     b += j->rank.n * 4 + j->mend.n * 8 + j->tmin.n * 4 + j->pre_n.n * 4 + j->pre_cnt.n * 4 + j->pre_nbrs.n * 4;
     for (const tsg_job::Ahead& A : j->ahead)
         b += A.rc.n * sizeof(uint2) + A.nbrs.n * 4 + A.cnt.n * 4 + A.n_at.n * 4 + A.mend.n * 8 + A.mnew.n * 8 +
-             A.tmin.n * 4;
+             A.tmin.n * 4 + A.dep.n * 8;
+    b += j->flow_flag.n * 4 + j->flow_ticket.n * 4;
     return b;
 }
 
@@ -1820,6 +1886,9 @@ void tsg_tuning_default(tsg_tuning* t) {
     t->ahead = 1;
     t->coop_wgs = 0;  // a grid barrier costs more than the launches it saves (measured)
     t->coop_max = 0;
+    t->flow = 1;  // 38.9 -> 27.9 ms on that workload
+    t->flow_wgs = 0;
+    t->flow_team = 4;
 }
 
 int tsg_job_set_tuning(tsg_job* job, const tsg_tuning* t) {
@@ -1829,6 +1898,8 @@ int tsg_job_set_tuning(tsg_job* job, const tsg_tuning* t) {
         return fail(TSG_ERR_INVALID_ARGUMENT, "team_max must be 1, 2, 4 or 8");
     if (t->coop_max > t->coop_wgs * small_batch_max())
         return fail(TSG_ERR_INVALID_ARGUMENT, "coop_max above 16 items per cooperative workgroup");
+    if (t->flow && (t->flow_team == 0 || t->flow_team > 8 || (t->flow_team & (t->flow_team - 1))))
+        return fail(TSG_ERR_INVALID_ARGUMENT, "flow_team must be 1, 2, 4 or 8");
     job->tune = *t;
     return TSG_OK;
 }

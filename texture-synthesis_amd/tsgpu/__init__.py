@@ -81,6 +81,9 @@ class Tuning(C.Structure):
         ("ahead", C.c_uint32),
         ("coop_wgs", C.c_uint32),
         ("coop_max", C.c_uint32),
+        ("flow", C.c_uint32),
+        ("flow_wgs", C.c_uint32),
+        ("flow_team", C.c_uint32),
     ]
 
 
