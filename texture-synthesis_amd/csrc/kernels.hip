@@ -79,10 +79,13 @@ struct WaveLds {
     uint32_t act_idx[kWave];  // index in the reference's candidate list
     float act_sum[kWave];
     double avg;
-    // team exchange: neighbour count (leader), best (cost, index) per member
+    // team exchange: neighbour count (leader), best (cost, index) per member,
+    // and (leader's copy) the team's least complete (cost, index) so far,
+    // packed as cost bits << 32 | index for a 64-bit minimum
     uint32_t t_kf;
     float t_B;
     uint32_t t_Bi;
+    unsigned long long t_bound;
 };
 
 // ---- k-smallest selection over canonical keys, one wavefront ----
@@ -522,11 +525,26 @@ struct Best {
 // reference's sequential f32 sum). A candidate whose partial sum already
 // exceeds the best complete cost (or equals it with a later index) can
 // never win and is dropped: partial sums of non-negative terms only grow.
+// The team's bound into `best` when it is lexicographically smaller; read by
+// one lane so the wave sees one value.
+__device__ __forceinline__ void take_team_bound(const unsigned long long* tb, Best& best) {
+    if (!tb) return;
+    const unsigned long long v = __hip_atomic_load(tb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    const uint32_t vi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t vb = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    const float B = __uint_as_float(vb);
+    if (vi != kNone && (best.Bi == kNone || B < best.B || (B == best.B && vi < best.Bi))) {
+        best.B = B;
+        best.Bi = vi;
+    }
+}
+
 template <bool GUIDED>
 __device__ __forceinline__ void eval_group(const MapDesc* maps, WaveLds* L, const float* lut, const float* glut, int slot, int lane,
-                           uint32_t kf, uint32_t n_act, Best& best) {
+                           uint32_t kf, uint32_t n_act, Best& best, unsigned long long* tb) {
     if (lane < (int)n_act) L->act_sum[lane] = 0.0f;
     wave_sync();
+    take_team_bound(tb, best);
     uint32_t j0 = 0;
     while (n_act > 0) {
         // chunk of 2^sh neighbours with n_act << sh <= kTermCap; while a bound
@@ -596,8 +614,13 @@ __device__ __forceinline__ void eval_group(const MapDesc* maps, WaveLds* L, cons
                 best.B = key;
                 best.Bi = ki;
             }
+            // publish: another member's complete cost bounds this one's candidates
+            if (tb && lane == 0 && best.Bi != kNone)
+                __hip_atomic_fetch_min(tb, ((unsigned long long)__float_as_uint(best.B) << 32) | best.Bi,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             break;
         }
+        take_team_bound(tb, best);
         const bool keep = act && !(s > best.B || (s == best.B && (best.Bi == kNone || myidx > best.Bi)));
         const uint64_t km = __ballot(keep);
         const uint32_t pos = (uint32_t)__popcll(km & lanemask_lt(lane));
@@ -680,35 +703,18 @@ __device__ __forceinline__ void item_at_random(const BatchArgs& a, const MapDesc
 // Every member: neighbour state (own pattern, distances, weights) into L and
 // the coherence candidates; returns n_coh. Lanes < n_k0 hold candidate
 // lane's (xy, map) in xy/ms; `uniq` marks the first of equal ones.
-template <bool GUIDED>
-__device__ __forceinline__ uint32_t item_neighbours(const BatchArgs& a, const MapDesc* maps, uint32_t item,
-                                                    uint32_t rel, int x, int y, uint32_t kf, uint32_t n_k0,
-                                                    WaveLds* L, int lane,
-                                                    uint32_t& xy, uint32_t& ms, uint64_t& uniq) {
-    bool coh_valid = false;
-    uint4 coh = make_uint4(0, 0, 0, 0);
+// The neighbour weights (ms.rs:405-425, 1198-1203), from the neighbour
+// offsets alone: distances, their average over the 4k duplicated entries
+// summed in order, exp(-d / avg).
+__device__ __forceinline__ void item_geometry(const BatchArgs& a, int x, int y, uint32_t kf, WaveLds* L, int lane) {
     if (lane < (int)kf) {
-        int32_t o = L->nd[lane];
-        int nx = x + packed_dx(o), ny = y + packed_dy(o);
-        uint32_t nflat = (uint32_t)ny * a.W + (uint32_t)nx;
-        uint4 st = a.state[nflat];
-        L->pat[lane] = st.x;
-        if (GUIDED) L->gpat[lane] = a.tg[nflat];
-        double dimx = (double)a.W, dimy = (double)a.H;
-        double x1 = (double)nx / dimx, y1 = (double)ny / dimy;
-        double x2 = (double)x / dimx, y2 = (double)y / dimy;
+        const int32_t o = L->nd[lane];
+        const int nx = x + packed_dx(o), ny = y + packed_dy(o);
+        const double dimx = (double)a.W, dimy = (double)a.H;
+        const double x1 = (double)nx / dimx, y1 = (double)ny / dimy;
+        const double x2 = (double)x / dimx, y2 = (double)y / dimy;
         L->dist[lane] = fma(x1 - x2, x1 - x2, (y1 - y2) * (y1 - y2));
-        if (!a.ext_cands) {
-            int cx = (int)(st.y & 0xffffu) - packed_dx(o);
-            int cy = (int)(st.y >> 16) - packed_dy(o);
-            uint32_t pmap = st.z >> 16;
-            coh_valid = pmap < a.n_maps && coord_ok(maps[pmap], cx, cy);
-            coh = make_uint4((uint32_t)cx | ((uint32_t)cy << 16), pmap, st.w, 0u);
-        }
     }
-    uint64_t cm = __ballot(coh_valid);
-    if (coh_valid) L->coh[__popcll(cm & lanemask_lt(lane))] = coh;
-    const uint32_t n_coh = (uint32_t)__popcll(cm);
     wave_sync();
     if (lane == 0) {
         // the reference's sum over the 4k duplicated distances, in order; the
@@ -741,6 +747,35 @@ __device__ __forceinline__ uint32_t item_neighbours(const BatchArgs& a, const Ma
     wave_sync();
     if (lane < (int)kf) L->w[lane] = (float)exp(-1.0 * (L->dist[lane] / L->avg));
     wave_sync();
+}
+
+template <bool GUIDED>
+__device__ __forceinline__ uint32_t item_neighbours(const BatchArgs& a, const MapDesc* maps, uint32_t item,
+                                                    uint32_t rel, int x, int y, uint32_t kf, uint32_t n_k0,
+                                                    WaveLds* L, int lane, bool geo_done,
+                                                    uint32_t& xy, uint32_t& ms, uint64_t& uniq) {
+    bool coh_valid = false;
+    uint4 coh = make_uint4(0, 0, 0, 0);
+    if (lane < (int)kf) {
+        int32_t o = L->nd[lane];
+        int nx = x + packed_dx(o), ny = y + packed_dy(o);
+        uint32_t nflat = (uint32_t)ny * a.W + (uint32_t)nx;
+        uint4 st = a.state[nflat];
+        L->pat[lane] = st.x;
+        if (GUIDED) L->gpat[lane] = a.tg[nflat];
+        if (!a.ext_cands) {
+            int cx = (int)(st.y & 0xffffu) - packed_dx(o);
+            int cy = (int)(st.y >> 16) - packed_dy(o);
+            uint32_t pmap = st.z >> 16;
+            coh_valid = pmap < a.n_maps && coord_ok(maps[pmap], cx, cy);
+            coh = make_uint4((uint32_t)cx | ((uint32_t)cy << 16), pmap, st.w, 0u);
+        }
+    }
+    uint64_t cm = __ballot(coh_valid);
+    if (coh_valid) L->coh[__popcll(cm & lanemask_lt(lane))] = coh;
+    const uint32_t n_coh = (uint32_t)__popcll(cm);
+    wave_sync();
+    if (!geo_done) item_geometry(a, x, y, kf, L, lane);
     // keep the first of equal (coordinate, map) coherence candidates: equal
     // patterns give equal costs and the first index wins ties (a host list
     // leads with its n_k0 coherence entries)
@@ -944,10 +979,23 @@ __device__ void resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t i
             rpre = a.rcands[(size_t)(item - a.rc_base) * a.m + r0 + (uint32_t)lane];
     }
     // ---- 1. k nearest resolved neighbours (leader) ----
+    // With a neighbour list computed ahead every member reads it and derives
+    // the weights at once (in a flow run, while the leader waits for the
+    // neighbours committed inside the run).
+    const bool pre = valid && !a.ext_nbrs && !a.knn_only && a.pre_nbrs && item >= a.pre_base && item < a.pre_end;
     if (valid && lead) {
         const uint32_t kf0 = item_knn(a, item, rel, x, y, n_res, L, lane);
-        if (lane == 0) L->t_kf = kf0;
+        if (lane == 0) {
+            L->t_kf = kf0;
+            L->t_bound = ~0ull;
+        }
+        if (pre && kf0) item_geometry(a, x, y, kf0, L, lane);
         if (flow) flow_wait(*flow, a, item, x, y, kf0, L, lane);
+    } else if (pre) {
+        const uint32_t kf0 = a.pre_cnt[item - a.pre_base];
+        if (lane < (int)kf0) L->nd[lane] = a.pre_nbrs[(size_t)(item - a.pre_base) * a.k + lane];
+        wave_sync();
+        if (kf0) item_geometry(a, x, y, kf0, L, lane);
     }
     if (T > 1) __syncthreads();
     else wave_sync();
@@ -955,7 +1003,7 @@ __device__ void resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t i
     uint32_t kf = 0;
     if (!done) {
         kf = Lt->t_kf;
-        if (!lead && lane < (int)kf) L->nd[lane] = Lt->nd[lane];
+        if (!lead && !pre && lane < (int)kf) L->nd[lane] = Lt->nd[lane];
         wave_sync();
         if (lead) phase_clock(a, item, 1, lane);
         if (kf == 0) {
@@ -969,7 +1017,7 @@ __device__ void resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t i
         // ---- 2. neighbour state, weights, coherence candidates (every member) ----
         uint32_t cxy, cms;
         uint64_t uniq;
-        n_coh = item_neighbours<GUIDED>(a, maps, item, rel, x, y, kf, n_k0, L, lane, cxy, cms, uniq);
+        n_coh = item_neighbours<GUIDED>(a, maps, item, rel, x, y, kf, n_k0, L, lane, pre, cxy, cms, uniq);
         if (!a.ext_cands) {
             n_k0 = n_coh;
             n_c = n_coh + a.m;
@@ -1026,7 +1074,7 @@ __device__ void resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t i
                     L->act_idx[lane] = ci;
                 }
                 wave_sync();
-                eval_group<GUIDED>(maps, L, lut, glut, slot, lane, kf, g_hi - g_lo, bst);
+                eval_group<GUIDED>(maps, L, lut, glut, slot, lane, kf, g_hi - g_lo, bst, T > 1 ? &Lt->t_bound : nullptr);
                 if (lead && first) phase_clock(a, item, 4, lane);
                 first = false;
                 g_lo = g_hi;
@@ -1251,6 +1299,7 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve_flow(BatchArgs a
         if (threadIdx.x == 0) {
             commit_one(a.W, a.tiles_x, maps, a.work, a.result, item, slot0, 0u, a.state_w, a.mask_w, a.score_w);
             __hip_atomic_store(f.flag + a.work[item], f.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            phase_clock(a, item, 10, 0);
         }
     }
     launch_span(a, t_start);
