@@ -19,6 +19,7 @@
 #include <string>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <thread>
 #include <vector>
 
@@ -45,25 +46,100 @@ int fail(int code, const std::string& msg) {
         }                                                                                          \
     } while (0)
 
+// Device blocks of destroyed jobs, per device, kept for the next jobs'
+// allocations: a job allocates about a hundred buffers (pyramids, state,
+// stage-ahead sets) and hipMalloc / hipFree cost tens of microseconds each.
+// Only releases made while g_pool_release is set (job destruction, when the
+// job has no device work in flight) enter the cache; any other release may
+// race queued kernels and goes to hipFree, which waits for them. At most
+// kPoolCap bytes per device stay cached; a block serves requests of half its
+// size and more.
+constexpr size_t kPoolCap = 4ull << 30;
+constexpr int kPoolDevices = 16;
+thread_local bool g_pool_release = false;
+
+struct DevPool {
+    std::mutex mu;
+    std::multimap<size_t, void*> free;  // bytes -> block
+    size_t cached = 0;
+};
+DevPool g_pool[kPoolDevices];
+
+DevPool* pool_here() {
+    int d = 0;
+    if (hipGetDevice(&d) != hipSuccess || d < 0 || d >= kPoolDevices) return nullptr;
+    return &g_pool[d];
+}
+
+// A cached block of at least `bytes` (and at most twice that), or a new one.
+hipError_t pool_get(size_t bytes, void** p, size_t* got) {
+    if (DevPool* pl = pool_here()) {
+        std::lock_guard<std::mutex> lk(pl->mu);
+        auto it = pl->free.lower_bound(bytes);
+        if (it != pl->free.end() && it->first <= 2 * bytes) {
+            *p = it->second;
+            *got = it->first;
+            pl->cached -= it->first;
+            pl->free.erase(it);
+            return hipSuccess;
+        }
+    }
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory) {
+        // give the cache back and retry once
+        if (DevPool* pl = pool_here()) {
+            std::lock_guard<std::mutex> lk(pl->mu);
+            for (auto& kv : pl->free) (void)hipFree(kv.second);
+            pl->free.clear();
+            pl->cached = 0;
+        }
+        e = hipMalloc(p, bytes);
+    }
+    if (e == hipSuccess) *got = bytes;
+    return e;
+}
+
+void pool_put(void* p, size_t bytes) {
+    if (g_pool_release) {
+        if (DevPool* pl = pool_here()) {
+            std::lock_guard<std::mutex> lk(pl->mu);
+            if (pl->cached + bytes <= kPoolCap) {
+                pl->free.emplace(bytes, p);
+                pl->cached += bytes;
+                return;
+            }
+        }
+    }
+    (void)hipFree(p);
+}
+
 // Owning device allocation.
 template <class T>
 struct DBuf {
     T* p = nullptr;
     size_t n = 0;
+    size_t bytes = 0;  // size of the block behind p
     DBuf() = default;
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
     ~DBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) pool_put(p, bytes);
         p = nullptr;
         n = 0;
+        bytes = 0;
     }
     hipError_t alloc(size_t count) {
         release();
         if (count == 0) count = 1;
-        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
-        if (e == hipSuccess) n = count;
+        void* q = nullptr;
+        size_t got = 0;
+        hipError_t e = pool_get(count * sizeof(T), &q, &got);
+        if (e == hipSuccess) {
+            p = static_cast<T*>(q);
+            n = count;
+            bytes = got;
+        }
         return e;
     }
     hipError_t reserve(size_t count) { return count <= n ? hipSuccess : alloc(count); }
@@ -1700,6 +1776,13 @@ void tsg_ctx_destroy(tsg_ctx* ctx) {
     if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     ctx->offtab.release();
     ctx->dummy_lut.release();
+    // the device's cached blocks are idle: give them back
+    if (DevPool* pl = pool_here()) {
+        std::lock_guard<std::mutex> lk(pl->mu);
+        for (auto& kv : pl->free) (void)hipFree(kv.second);
+        pl->free.clear();
+        pl->cached = 0;
+    }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
     delete ctx;
@@ -2030,8 +2113,14 @@ void tsg_job_destroy(tsg_job* job) {
     if (!job) return;
     (void)hipSetDevice(job->ctx->device);
     // a finished job has no work in flight (downloads wait for theirs)
-    if (job->step.active) (void)hipStreamSynchronize(job->ctx->stream);
+    if (job->step.active) {
+        (void)hipStreamSynchronize(job->ctx->stream);
+        (void)hipStreamSynchronize(job->ctx->side);
+    }
+    // its blocks go to the device's cache
+    g_pool_release = true;
     delete job;
+    g_pool_release = false;
 }
 
 int tsg_pyramid_build(tsg_ctx* ctx, const tsg_image* img, uint32_t levels, uint8_t* out) {

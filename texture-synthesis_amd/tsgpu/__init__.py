@@ -426,6 +426,29 @@ class Session:
             L.tsg_job_destroy(self._job)
             self._job = None
 
+    def run_resident(self) -> "Session":
+        """Runs the job and keeps its maps in device memory; download() then
+        copies them out and close() frees the job."""
+        if self._job is None or getattr(self, "_ran", False):
+            raise RuntimeError("a session runs once")
+        try:
+            _check(lib().tsg_job_run(self._job, PROGRESS_FN(), None))
+        except Exception:
+            self.close()
+            raise
+        self._ran = True
+        return self
+
+    def download(self) -> GeneratedImage:
+        if self._job is None or not getattr(self, "_ran", False):
+            raise RuntimeError("download() needs a job finished by run_resident()")
+        return self._collect()
+
+    def close(self) -> None:
+        if self._job is not None:
+            lib().tsg_job_destroy(self._job)
+            self._job = None
+
     def footprint(self) -> int:
         """Device bytes the job holds now (inputs, state and scratch)."""
         v = C.c_uint64()
