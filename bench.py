@@ -182,6 +182,7 @@ def serving_leg(ts, synth, n, qps, budget_gib):
     srv = serve.Server(ts.Context.default(), hbm_budget=int(budget_gib * (1 << 30)), max_active=256,
                        rounds_per_pump=4)
     fin = []
+    busy = 0.0  # wall time with at least one request admitted or waiting
     t0 = time.perf_counter()
     i = 0
     while i < n or not srv.idle():
@@ -192,7 +193,9 @@ def serving_leg(ts, synth, n, qps, budget_gib):
         if srv.idle():
             time.sleep(max(0.0, min(0.01, arrivals[i] - now)))
             continue
+        ta = time.perf_counter()
         fin.extend(srv.pump())
+        busy += time.perf_counter() - ta
     t_end = time.perf_counter()
     srv.close()
     errors = [f for f in fin if f.error is not None]
@@ -206,6 +209,7 @@ def serving_leg(ts, synth, n, qps, budget_gib):
                        "max": round(max(lat), 1) if lat else None},
         "offered_mpix_per_s": round(sum(r[1] for r in reqs) / 1e6 * qps / n, 3),
         "seconds": round(t_end - t0, 3), "errors": len(errors),
+        "busy_fraction": round(busy / (t_end - t0), 3),
         "hbm_budget_gib": budget_gib, "peak_admitted_gib": round(srv.peak_in_use / (1 << 30), 3),
         "memory_holds": srv.memory_holds,
     }
@@ -249,8 +253,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched", type=int, default=256, help="syntheses in the batched leg (0 = skip)")
     ap.add_argument("--guided", type=int, default=3, help="runs of the guided leg (0 = skip)")
-    ap.add_argument("--serve-requests", type=int, default=120, help="requests in the serving leg (0 = skip)")
-    ap.add_argument("--serve-qps", type=float, default=12.0)
+    ap.add_argument("--serve-requests", type=int, default=480, help="requests in the serving leg (0 = skip)")
+    ap.add_argument("--serve-qps", type=float, default=18.0,
+                    help="Poisson arrival rate; 18/s keeps the engine busy about 70%% of the time")
     ap.add_argument("--serve-budget-gib", type=float, default=4.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration)")
