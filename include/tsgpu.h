@@ -50,7 +50,8 @@ typedef struct tsg_params {
     uint32_t backtrack_stages;        /* P. default 5 */
     float guide_alpha;                /* [0,1]. default 0.8 */
     uint64_t seed;                    /* default 0 */
-    uint32_t tiling_mode;             /* 0/1. default 0 */
+    uint32_t tiling_mode;             /* 0/1. default 0. 1: ghost points at the far edge within 5%
+                                       * margins (ms.rs:296-331), wrapped own pattern */
     uint64_t random_init;             /* UINT64_MAX = none (random_resolve None) */
     uint32_t max_thread_count;        /* accepted for API parity (>= 1); the device ignores it */
 } tsg_params;

@@ -59,6 +59,12 @@ struct BatchArgs {
     const int32_t* pre_nbrs;      // [item - pre_base][k] packed dx | dy << 16
     const uint32_t* pre_cnt;
     uint32_t pre_base, pre_end;
+    // tiling mode (null gmask: off): per 8x8 tile, the resolved pixels that
+    // carry ghost copies one image width / height away, on the far side, while
+    // within the edge margins x < xl, x > xr, y < yb, y > yt; neighbour records
+    // and the own pattern wrap around the image
+    uint64_t* gmask;
+    int32_t xl, xr, yb, yt;
 };
 
 // kNN of stage items [item0, item1) ahead of their batches: item i sees the
@@ -126,6 +132,7 @@ struct CommitArgs {
     uint4* state;
     uint64_t* mask;
     float* score;            // per pixel score of the first resolve
+    uint64_t* gmask;         // tiling mode: ghost flags of new pixels (null: off)
 };
 
 struct RecolorArgs {

@@ -210,6 +210,10 @@ __device__ __forceinline__ bool px_set(const uint64_t* mask, uint32_t tiles_x, i
 // 8x8 tile), resolved list, offset table. With `mask_new`, a pixel set there
 // also counts when rank[p] < n_at (resolved before an earlier moment);
 // tile_min[t] is the least rank among tile t's mask_new pixels.
+// In tiling mode (gmask set) the points also include the ghost copies of
+// flush_resolved (ms.rs:296-331): a pixel with its gmask bit set and x < xl
+// (x > xr) also stands at x + W (x - W); with y < yb (y > yt) at y + H
+// (y - H); there are no diagonal copies.
 struct KnnView {
     uint32_t W, H, tiles_x, tiles_y;
     const uint64_t* mask;
@@ -220,7 +224,64 @@ struct KnnView {
     const uint32_t* tile_min;
     const uint32_t* rank;
     uint32_t n_at;
+    const uint64_t* gmask;
+    int32_t xl, xr, yb, yt;
 };
+
+// Whether a point stands at (X, Y), ghosts included (no mask_new).
+__device__ __forceinline__ bool point_at(const KnnView& v, int X, int Y) {
+    const bool xin = X >= 0 && X < (int)v.W, yin = Y >= 0 && Y < (int)v.H;
+    if (xin && yin) return px_set(v.mask, v.tiles_x, X, Y);
+    if (!v.gmask || (!xin && !yin)) return false;
+    if (!xin) {
+        const int x0 = X >= (int)v.W ? X - (int)v.W : X + (int)v.W;
+        const bool ok = X >= (int)v.W ? x0 < v.xl : x0 > v.xr;
+        return ok && x0 >= 0 && x0 < (int)v.W && px_set(v.gmask, v.tiles_x, x0, Y);
+    }
+    const int y0 = Y >= (int)v.H ? Y - (int)v.H : Y + (int)v.H;
+    const bool ok = Y >= (int)v.H ? y0 < v.yb : y0 > v.yt;
+    return ok && y0 >= 0 && y0 < (int)v.H && px_set(v.gmask, v.tiles_x, X, y0);
+}
+
+// Ghost flags of pixels (xs + i, Y), i = 0..7, as bits (0 outside [0, W)).
+__device__ __forceinline__ uint32_t ghost_row(const KnnView& v, int Y, int xs) {
+    const int c0 = xs >> 3, off = xs - c0 * 8, sh = (Y & 7) * 8;
+    const uint32_t row = (uint32_t)(Y >> 3) * v.tiles_x;
+    uint32_t b0 = 0, b1 = 0;
+    if (c0 >= 0 && c0 < (int)v.tiles_x) b0 = (uint32_t)(v.gmask[row + (uint32_t)c0] >> sh) & 0xffu;
+    if (c0 + 1 >= 0 && c0 + 1 < (int)v.tiles_x) b1 = (uint32_t)(v.gmask[row + (uint32_t)(c0 + 1)] >> sh) & 0xffu;
+    return ((b0 | (b1 << 8)) >> off) & 0xffu;
+}
+
+// Bits i of [lo, hi) within 0..7.
+__device__ __forceinline__ uint32_t bit_range(int lo, int hi) {
+    lo = max(lo, 0);
+    hi = min(hi, 8);
+    return hi > lo ? ((1u << hi) - 1u) & ~((1u << lo) - 1u) : 0u;
+}
+
+// The points of 8x8 tile (gx, gy) as a tile word; in tiling mode the tile
+// may lie partly or wholly outside the image and holds ghost copies there.
+__device__ __forceinline__ uint64_t tile_points(const KnnView& v, int gx, int gy, bool& in_grid) {
+    in_grid = gx >= 0 && gy >= 0 && gx < (int)v.tiles_x && gy < (int)v.tiles_y;
+    uint64_t w = in_grid ? v.mask[(uint32_t)gy * v.tiles_x + (uint32_t)gx] : 0ull;
+    const int X0 = gx * 8, Y0 = gy * 8, W = (int)v.W, H = (int)v.H;
+    if (!v.gmask || (X0 >= 0 && Y0 >= 0 && X0 + 8 <= W && Y0 + 8 <= H)) return w;
+    for (int j = 0; j < 8; ++j) {
+        const int Y = Y0 + j;
+        uint32_t bits = 0;
+        if (Y >= 0 && Y < H) {
+            if (X0 + 8 > W) bits |= ghost_row(v, Y, X0 - W) & bit_range(W - X0, v.xl + W - X0);
+            if (X0 < 0) bits |= ghost_row(v, Y, X0 + W) & bit_range(v.xr + 1 - X0 - W, -X0);
+        } else if (Y >= H) {
+            if (Y - H < v.yb) bits = ghost_row(v, Y - H, X0) & bit_range(-X0, W - X0);
+        } else if (Y + H > v.yt) {
+            bits = ghost_row(v, Y + H, X0) & bit_range(-X0, W - X0);
+        }
+        w |= (uint64_t)bits << (j * 8);
+    }
+    return w;
+}
 
 // The pixels of tile (gx, gy)'s new-pixel word nw that were resolved before
 // v.n_at; rank loads are issued eight at a time.
@@ -263,9 +324,19 @@ __device__ __forceinline__ void knn_list(const KnnView& a, WaveLds* L, uint32_t 
             const uint32_t idx = base + u * kWave + lane;
             if (base + u * kWave >= n_res) break;
             const bool v = idx < n_res;
+            const int px = (int)(f[u] % a.W), py = (int)(f[u] / a.W);
             uint64_t key = ~0ull;
-            if (v) key = make_key((int)(f[u] % a.W) - x, (int)(f[u] / a.W) - y);
+            if (v) key = make_key(px - x, py - y);
             sel_push(L->keys, s, k, v, key, lane);
+            if (a.gmask) {
+                // the point's ghost copies (flush_resolved order: x, then y)
+                const bool g = v && px_set(a.gmask, a.tiles_x, px, py);
+                const bool gx = g && (px < a.xl || px > a.xr);
+                const bool gy = g && (py < a.yb || py > a.yt);
+                const int sx = px < a.xl ? (int)a.W : -(int)a.W, sy = py < a.yb ? (int)a.H : -(int)a.H;
+                sel_push(L->keys, s, k, gx, gx ? make_key(px + sx - x, py - y) : ~0ull, lane);
+                sel_push(L->keys, s, k, gy, gy ? make_key(px - x, py + sy - y) : ~0ull, lane);
+            }
         }
     }
 }
@@ -310,8 +381,15 @@ __device__ __forceinline__ void spiral_tile(int t, int& ox, int& oy) {
 __device__ __forceinline__ void knn_rings(const KnnView& a, WaveLds* L, int x, int y, uint32_t k, int lane, Sel& s,
                                           uint32_t& iters) {
     const int tx = x >> 3, ty = y >> 3;
-    const int TX = (int)a.tiles_x, TY = (int)a.tiles_y;
-    const int rmax = max(max(tx, TX - 1 - tx), max(ty, TY - 1 - ty));
+    // tiles that can hold points: the grid, plus the ghost margins in tiling mode
+    int g_x0 = 0, g_y0 = 0, g_x1 = (int)a.tiles_x - 1, g_y1 = (int)a.tiles_y - 1;
+    if (a.gmask) {
+        g_x0 = (1 - a.xl) >> 3;
+        g_y0 = (1 - a.yb) >> 3;
+        g_x1 = max(g_x1, ((int)a.W + a.xl - 1) >> 3);
+        g_y1 = max(g_y1, ((int)a.H + a.yb - 1) >> 3);
+    }
+    const int rmax = max(max(tx - g_x0, g_x1 - tx), max(ty - g_y0, g_y1 - ty));
     const int t_end = (2 * rmax + 1) * (2 * rmax + 1);
     iters = 0;
     for (int t0 = 0; t0 < t_end; t0 += 4 * kWave) {
@@ -331,10 +409,11 @@ __device__ __forceinline__ void knn_rings(const KnnView& a, WaveLds* L, int x, i
                 spiral_tile(t, ox, oy);
                 gx[u] = tx + ox;
                 gy[u] = ty + oy;
-                if (gx[u] >= 0 && gy[u] >= 0 && gx[u] < TX && gy[u] < TY) {
-                    const uint32_t ti = (uint32_t)gy[u] * a.tiles_x + (uint32_t)gx[u];
-                    word[u] = a.mask[ti];
-                    if (a.mask_new) {
+                if (gx[u] >= g_x0 && gy[u] >= g_y0 && gx[u] <= g_x1 && gy[u] <= g_y1) {
+                    bool in_grid;
+                    word[u] = tile_points(a, gx[u], gy[u], in_grid);
+                    if (a.mask_new && in_grid) {
+                        const uint32_t ti = (uint32_t)gy[u] * a.tiles_x + (uint32_t)gx[u];
                         nword[u] = a.mask_new[ti];
                         tmin[u] = a.tile_min[ti];
                     }
@@ -365,10 +444,10 @@ __device__ __forceinline__ void knn_rings(const KnnView& a, WaveLds* L, int x, i
         if (rc >= rmax) break;
         if (s.count >= k && rc >= 0) {
             int64_t D = INT64_MAX, e;
-            if (tx - rc > 0 && (e = x - (tx - rc) * 8 + 1) < D) D = e;
-            if (tx + rc < TX - 1 && (e = (tx + rc + 1) * 8 - x) < D) D = e;
-            if (ty - rc > 0 && (e = y - (ty - rc) * 8 + 1) < D) D = e;
-            if (ty + rc < TY - 1 && (e = (ty + rc + 1) * 8 - y) < D) D = e;
+            if (tx - rc > g_x0 && (e = x - (tx - rc) * 8 + 1) < D) D = e;
+            if (tx + rc < g_x1 && (e = (tx + rc + 1) * 8 - x) < D) D = e;
+            if (ty - rc > g_y0 && (e = y - (ty - rc) * 8 + 1) < D) D = e;
+            if (ty + rc < g_y1 && (e = (ty + rc + 1) * 8 - y) < D) D = e;
             const uint64_t lim = (uint64_t)(D * D) << 32;
             uint32_t c = 0;
             for (int q = lane; q < (int)s.count; q += kWave) c += L->keys[q] < lim;
@@ -396,7 +475,7 @@ __device__ __forceinline__ uint32_t knn_table(const KnnView& a, WaveLds* L, int 
                 o[u] = a.offtab[t];
                 const int X = x + packed_dx(o[u]), Y = y + packed_dy(o[u]);
                 const bool in = X >= 0 && Y >= 0 && X < (int)a.W && Y < (int)a.H;
-                hit[u] = in && px_set(a.mask, a.tiles_x, X, Y);
+                hit[u] = in ? px_set(a.mask, a.tiles_x, X, Y) : point_at(a, X, Y);
                 if (in && !hit[u] && a.mask_new && px_set(a.mask_new, a.tiles_x, X, Y))
                     hit[u] = a.rank[(uint32_t)Y * a.W + (uint32_t)X] < a.n_at;
             }
@@ -423,7 +502,9 @@ __device__ __forceinline__ uint32_t knn_search(const KnnView& v, WaveLds* L, int
     const bool sparse = (uint64_t)n_res * 1024ull < (uint64_t)k * v.W * v.H;
     if (!use_sel && !sparse) {
         found = knn_table(v, L, x, y, k, lane);
-        use_sel = found < k && found < n_res;
+        // fewer than k within the table: search on, unless every point was
+        // found (without ghosts there are n_res of them)
+        use_sel = found < k && (v.gmask || found < n_res);
     } else if (!use_sel) {
         use_sel = true;
     }
@@ -669,7 +750,8 @@ __device__ __forceinline__ uint32_t item_knn(const BatchArgs& a, uint32_t item, 
         kf = a.pre_cnt[item - a.pre_base];
         if (lane < (int)kf) L->nd[lane] = a.pre_nbrs[(size_t)(item - a.pre_base) * k + lane];
     } else {
-        const KnnView v{a.W, a.H, a.tiles_x, a.tiles_y, a.mask, a.rlist, a.offtab, a.offtab_n, nullptr, nullptr, nullptr, 0u};
+        const KnnView v{a.W,     a.H,     a.tiles_x, a.tiles_y, a.mask, a.rlist, a.offtab, a.offtab_n, nullptr,
+                        nullptr, nullptr, 0u,        a.gmask,   a.xl,   a.xr,    a.yb,     a.yt};
         uint32_t iters, buffered;
         kf = knn_search(v, L, x, y, k, n_res, lane, iters, buffered);
         phase_clock(a, item, 7, lane);
@@ -759,6 +841,11 @@ __device__ __forceinline__ uint32_t item_neighbours(const BatchArgs& a, const Ma
     if (lane < (int)kf) {
         int32_t o = L->nd[lane];
         int nx = x + packed_dx(o), ny = y + packed_dy(o);
+        if (a.gmask) {
+            // a ghost neighbour reads the pixel it copies (ms.rs:510-512, 1132-1136)
+            nx = nx < 0 ? nx + (int)a.W : (nx >= (int)a.W ? nx - (int)a.W : nx);
+            ny = ny < 0 ? ny + (int)a.H : (ny >= (int)a.H ? ny - (int)a.H : ny);
+        }
         uint32_t nflat = (uint32_t)ny * a.W + (uint32_t)nx;
         uint4 st = a.state[nflat];
         L->pat[lane] = st.x;
@@ -1117,9 +1204,12 @@ __device__ void resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t i
 
 // Generator::update (ms.rs:334-377) for one item: colour from the stage
 // level, source coordinate, ids; new pixels also enter the resolved mask.
+// In tiling mode (gmask) a new pixel resolved from neighbours also gets its
+// ghost copies (flush_resolved with tiling, ms.rs:296-331); one resolved at
+// random does not.
 __device__ __forceinline__ void commit_one(uint32_t W, uint32_t tiles_x, const MapDesc* maps, const uint32_t* work,
                                            const uint4* result, uint32_t item, uint32_t item0, uint32_t first_new,
-                                           uint4* state, uint64_t* mask, float* score) {
+                                           uint4* state, uint64_t* mask, float* score, uint64_t* gmask) {
     uint32_t p = work[item];
     uint4 r = result[item - item0];
     uint32_t cmap = r.y & 0xffffu;
@@ -1129,7 +1219,10 @@ __device__ __forceinline__ void commit_one(uint32_t W, uint32_t tiles_x, const M
     state[p] = make_uint4(color, r.x, r.y & ~kFallbackBit, r.z);
     if (item >= first_new) {
         uint32_t px = p % W, py = p / W;
-        atomicOr((unsigned long long*)&mask[(py >> 3) * tiles_x + (px >> 3)], 1ull << (((py & 7u) << 3) | (px & 7u)));
+        const uint32_t ti = (py >> 3) * tiles_x + (px >> 3);
+        const uint64_t bit = 1ull << (((py & 7u) << 3) | (px & 7u));
+        atomicOr((unsigned long long*)&mask[ti], bit);
+        if (gmask && !(r.y & kFallbackBit)) atomicOr((unsigned long long*)&gmask[ti], bit);
         score[p] = __uint_as_float(r.w);
     }
 }
@@ -1195,7 +1288,7 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve(BatchArgs a) {
         __threadfence();
         for (uint32_t i = a.item0 + threadIdx.x; i < a.item1; i += blockDim.x)
             commit_one(a.W, a.tiles_x, maps, a.work, a.result, i, a.item0, a.first_new, a.state_w, a.mask_w,
-                       a.score_w);
+                       a.score_w, a.gmask);
         if (threadIdx.x == 0) *a.done_counter = 0u;
     }
     launch_span(a, t_start);
@@ -1226,7 +1319,7 @@ __global__ __launch_bounds__(kWave* kSmallWaves) void k_resolve_small(BatchArgs 
         __syncthreads();
         const uint32_t ci = i0 + threadIdx.x;
         if (ci < i1) commit_one(a.W, a.tiles_x, maps, a.work, a.result, ci, i0, a.first_new, a.state_w, a.mask_w,
-                                a.score_w);
+                                a.score_w, a.gmask);
         __syncthreads();
     }
     launch_span(a, t_start);
@@ -1266,7 +1359,7 @@ __global__ __launch_bounds__(kWave* kSmallWaves) void k_resolve_coop(BatchArgs a
         if (n <= waves)
             for (uint32_t ci = i0 + blockIdx.x * blockDim.x + threadIdx.x; ci < i1; ci += gridDim.x * blockDim.x)
                 commit_one(a.W, a.tiles_x, maps, a.work, a.result, ci, i0, a.first_new, a.state_w, a.mask_w,
-                           a.score_w);
+                           a.score_w, a.gmask);
         grid.sync();
     }
 }
@@ -1297,7 +1390,8 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve_flow(BatchArgs a
         const uint32_t slot0 = item - blockIdx.x;  // result[item - slot0] is this workgroup's slot
         resolve_team<GUIDED>(a, maps, item, slot0, 0u, wls, wave, T, true, lut, glut, lane, &f);
         if (threadIdx.x == 0) {
-            commit_one(a.W, a.tiles_x, maps, a.work, a.result, item, slot0, 0u, a.state_w, a.mask_w, a.score_w);
+            commit_one(a.W, a.tiles_x, maps, a.work, a.result, item, slot0, 0u, a.state_w, a.mask_w, a.score_w,
+                       a.gmask);
             __hip_atomic_store(f.flag + a.work[item], f.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             phase_clock(a, item, 10, 0);
         }
@@ -1346,7 +1440,7 @@ __global__ void k_commit_multi(const BatchArgs* units, const uint32_t* item_star
     const uint32_t u = find_unit(item_start, n_units, g);
     const BatchArgs& a = units[u];
     commit_one(a.W, a.tiles_x, a.maps, a.work, a.result, a.item0 + (g - item_start[u]), a.item0, a.first_new,
-               a.state_w, a.mask_w, a.score_w);
+               a.state_w, a.mask_w, a.score_w, a.gmask);
 }
 
 // Neighbour lists of items [item0, item1) ahead of their batches, one
@@ -1450,7 +1544,8 @@ __global__ void k_rand_multi(const RandArgs* units, const uint32_t* item_start, 
 __global__ void k_commit(CommitArgs a) {
     uint32_t item = a.item0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (item >= a.item1) return;
-    commit_one(a.W, a.tiles_x, a.maps, a.work, a.result, item, a.item0, a.first_new, a.state, a.mask, a.score);
+    commit_one(a.W, a.tiles_x, a.maps, a.work, a.result, item, a.item0, a.first_new, a.state, a.mask, a.score,
+               a.gmask);
 }
 
 // next_pyramid_level (ms.rs:687-700): recolour every resolved pixel.
@@ -1722,7 +1817,7 @@ hipError_t launch_resolve_flow(const BatchArgs& a, const FlowArgs& f, uint32_t w
         return hipErrorInvalidValue;
     // every item needs its neighbour list; test hooks are not supported
     if (!a.pre_nbrs || a.item0 < a.pre_base || a.item0 + f.n > a.pre_end || a.ext_nbrs || a.ext_cands ||
-        a.dbg_nbrs || a.dbg_best || a.knn_only || a.ext_full_cost)
+        a.dbg_nbrs || a.dbg_best || a.knn_only || a.ext_full_cost || a.gmask)
         return hipErrorInvalidValue;
     const uint32_t lds = (uint32_t)flow_lds_bytes(team, guided);
     if (guided) {

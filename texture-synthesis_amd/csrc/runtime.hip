@@ -502,6 +502,7 @@ struct tsg_job {
     // device state
     DBuf<uint4> state;
     DBuf<uint64_t> mask;
+    DBuf<uint64_t> gmask;  // tiling mode: pixels with ghost copies (ms.rs:296-331)
     DBuf<float> score;
     DBuf<uint32_t> rlist;
     // scratch
@@ -730,6 +731,8 @@ int init_state(tsg_job* j) {
     hipStream_t s = j->ctx->stream;
     HIP_TRY(hipMemcpyAsync(j->state.p, st.data(), N * sizeof(uint4), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(j->mask.p, mask.data(), mask.size() * 8, hipMemcpyHostToDevice, s));
+    // every pixel resolved before the stages starts with its ghost copies (ms.rs:296-331)
+    if (j->prm.tiling_mode) HIP_TRY(hipMemcpyAsync(j->gmask.p, mask.data(), mask.size() * 8, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemsetAsync(j->score.p, 0, N * sizeof(float), s));
     if (!rl.empty()) HIP_TRY(hipMemcpyAsync(j->rlist.p, rl.data(), rl.size() * 4, hipMemcpyHostToDevice, s));
     std::vector<uint32_t> rank(N, UINT32_MAX);
@@ -1102,6 +1105,14 @@ int step_open_stage(tsg_job* j) {
     a.mask_w = j->mask.p;
     a.score_w = j->score.p;
     a.result = j->result.p;
+    if (j->prm.tiling_mode) {
+        // ghost margins: 5% of each side (ms.rs:296-331)
+        a.gmask = j->gmask.p;
+        a.xl = (int32_t)((float)j->W * 0.05f);
+        a.xr = (int32_t)j->W - a.xl;
+        a.yb = (int32_t)((float)j->H * 0.05f);
+        a.yt = (int32_t)j->H - a.yb;
+    }
     if (j->ahead_on) {
         // every item of the stage has its candidates and neighbour list
         a.rcands = j->ahead[t % 2].rc.p;
@@ -1158,7 +1169,8 @@ bool step_peek(tsg_job* j, uint64_t& i0, uint64_t& i1) {
 // computed yet.
 bool step_wants_prefetch(const tsg_job* j, uint64_t i0, uint64_t i1) {
     const BatchArgs& a = j->step.a;
-    if (j->ahead_on) return false;
+    // tiling mode searches in-kernel: a window's picks may or may not get ghost copies
+    if (j->ahead_on || j->prm.tiling_mode) return false;
     if (i1 - i0 > j->tune.prefetch_max) return false;
     return !(a.pre_nbrs && i0 >= a.pre_base && i1 <= a.pre_end);
 }
@@ -1310,7 +1322,7 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
     if ((rc = step_begin(j))) return rc;
     // content-independent work a stage ahead on the side stream when two
     // buffer sets of the largest stage fit the budget
-    j->ahead_on = j->tune.ahead && ahead_bytes(j) <= kAheadBudget;
+    j->ahead_on = j->tune.ahead && !j->prm.tiling_mode && ahead_bytes(j) <= kAheadBudget;
     if (j->ahead_on) {
         const uint64_t smax = std::max<uint64_t>(1, stage_size(j, 0));
         for (tsg_job::Ahead& A : j->ahead) {
@@ -1457,8 +1469,8 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
                 a.team = team_for(i1 - i0, j->tune.team_max, j->tune.team_waves);
                 HIP_TRY(timed_launch(j, K_RESOLVE, i1 - i0, [&] { return launch_resolve(a, j->guided, s); }));
                 if (!a.fused_commit) {
-                    CommitArgs ca{j->W, j->tiles_x, a.maps, a.work, j->result.p, (uint32_t)i0, (uint32_t)i1,
-                                  a.first_new, j->state.p, j->mask.p, j->score.p};
+                    CommitArgs ca{j->W,       j->tiles_x, a.maps,    a.work,      j->result.p, (uint32_t)i0, (uint32_t)i1,
+                                  a.first_new, j->state.p, j->mask.p, j->score.p, a.gmask};
                     HIP_TRY(timed_launch(j, K_COMMIT, i1 - i0, [&] { return launch_commit(ca, s); }));
                 }
             }
@@ -1483,7 +1495,7 @@ size_t job_footprint(const tsg_job* j) {
     for (auto& e : j->exg) b += e->n * 4;
     for (auto& e : j->smask) b += e->n;
     b += j->tg.n * 4 + j->maps.n * sizeof(MapDesc);
-    b += j->N * (sizeof(uint4) + 4 + 4 + 4) + (size_t)j->tiles_x * j->tiles_y * 8;
+    b += j->N * (sizeof(uint4) + 4 + 4 + 4) + (size_t)j->tiles_x * j->tiles_y * 8 + j->gmask.n * 8;
     b += j->result.n * sizeof(uint4) + j->rc.n * sizeof(uint2) + j->small_tab.n * 4;
     b += j->rank.n * 4 + j->mend.n * 8 + j->tmin.n * 4 + j->pre_n.n * 4 + j->pre_cnt.n * 4 + j->pre_nbrs.n * 4;
     for (const tsg_job::Ahead& A : j->ahead)
@@ -1813,7 +1825,6 @@ int tsg_job_create(tsg_ctx* ctx, const tsg_params* prm, uint32_t n_examples, con
     if (prm->nearest_neighbors == 0 || prm->nearest_neighbors > (uint32_t)kMaxK)
         return fail(TSG_ERR_UNSUPPORTED, "nearest_neighbors must be in 1..64 on the device");
     if (prm->random_sample_locations > 4096) return fail(TSG_ERR_UNSUPPORTED, "random_sample_locations above 4096");
-    if (prm->tiling_mode) return fail(TSG_ERR_UNSUPPORTED, "tiling mode is not on the device path yet");
     if (target_guide && !example_guides)
         return fail(TSG_ERR_UNSUPPORTED, "style transfer (target guide without example guides) needs host guide prep");
     if (out_w == 0 || out_h == 0 || out_w > 16384 || out_h > 16384)
@@ -1935,6 +1946,7 @@ int tsg_job_create(tsg_ctx* ctx, const tsg_params* prm, uint32_t n_examples, con
     const uint32_t P = prm->backtrack_stages;
     HIP_TRY(j->state.alloc(j->N));
     HIP_TRY(j->mask.alloc((size_t)j->tiles_x * j->tiles_y));
+    if (prm->tiling_mode) HIP_TRY(j->gmask.alloc((size_t)j->tiles_x * j->tiles_y));
     HIP_TRY(j->score.alloc(j->N));
     HIP_TRY(j->rlist.alloc(j->N));
     HIP_TRY(j->item_score.alloc(j->N));
@@ -2438,8 +2450,8 @@ int tsg_footprint_estimate(const tsg_params* prm, uint32_t n_examples, const uin
     }
     if (guide_wh) b += levels * N * 4;
     b += levels * nf * sizeof(MapDesc);
-    // state, score, resolved list, per-item scores, resolved-mask tiles
-    b += N * (sizeof(uint4) + 12) + (uint64_t)((out_w + 7) / 8) * ((out_h + 7) / 8) * 8;
+    // state, score, resolved list, per-item scores, resolved-mask tiles (and ghost flags)
+    b += N * (sizeof(uint4) + 12) + (uint64_t)((out_w + 7) / 8) * ((out_h + 7) / 8) * (prm->tiling_mode ? 16 : 8);
     // scratch under the engine: largest batch results, one random-candidate
     // chunk, per-stage statistics (default schedule: largest batch N >> 5)
     const uint64_t maxb = std::min<uint64_t>(N, std::max<uint64_t>((N >> 5) + 1, small_batch_max()));
