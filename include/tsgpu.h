@@ -83,9 +83,13 @@ typedef struct tsg_schedule {
  *   prefetch_max batches of at most this many items get their neighbour
  *                lists computed ahead, window by window (0 = never)
  *   ahead        1: a single-job run prepares each stage's content-independent
- *                work (random candidates, pick ranks, every neighbour list)
- *                during the previous stage on a second stream, when it fits
- *                in device memory; 0: per window on the job stream
+ *                work (random candidates, pick ranks, the neighbour lists of
+ *                its leading latency-bound batches) during the previous
+ *                stage on a second stream, in two buffer sets used in turn,
+ *                when they fit in device memory; 2: one buffer set per stage,
+ *                every item's neighbour list, prepared as far ahead as the
+ *                host has reached (when the stages fit, else as 1); 0: per
+ *                window on the job stream
  *   coop_wgs     workgroups (16 waves each) of the cooperative launch that
  *                runs consecutive latency-bound batches with grid barriers
  *                instead of two launches per batch (0 = off)

@@ -239,6 +239,8 @@ def test_default_schedule_bit_exact_c1(ctx, seed):
     dict(coop_wgs=8, coop_max=128, ahead=0),
     dict(coop_wgs=256, coop_max=4096, team_max=8, flow=0),
     dict(flow=0),
+    dict(flow=0, ahead=2),
+    dict(ahead=1),
     dict(flow=1, flow_team=1, flow_wgs=3),
     dict(flow=1, flow_team=2, flow_wgs=1),
     dict(flow=1, flow_team=4, prefetch_max=64),

@@ -80,7 +80,10 @@ struct PrefetchArgs {
     const uint32_t* rank;
     const uint32_t* rlist;
     const uint32_t* work;
-    const uint32_t* n_at;
+    const uint32_t* n_at;         // [item - item0], or (null) from the batch table:
+    const uint32_t* bounds;       // batch b holds items [bounds[b], bounds[b + 1]) (nb + 1 entries)
+    const uint32_t* bnres;        // and starts with bnres[b] resolved pixels
+    uint32_t nb;
     const int32_t* offtab;
     uint32_t offtab_n;
     uint32_t k;

@@ -1453,7 +1453,18 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_knn_prefetch(PrefetchArg
     if (item >= a.item1) return;
     const uint32_t rel = item - a.item0;
     const uint32_t flat = a.work[item];
-    const uint32_t n_at = a.n_at[rel];
+    uint32_t n_at;
+    if (a.n_at) {
+        n_at = a.n_at[rel];
+    } else {
+        uint32_t lo = 0, hi = a.nb - 1;  // last batch starting at or before the item
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (a.bounds[mid] <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        n_at = a.bnres[lo];
+    }
     const KnnView v{a.W,        a.H,      a.tiles_x, a.tiles_y, a.mask, a.rlist, a.offtab, a.offtab_n,
                     a.mask_new, a.tile_min, a.rank, n_at};
     uint32_t iters, buffered;
@@ -1677,6 +1688,7 @@ hipError_t launch_knn_prefetch(const PrefetchArgs& a, hipStream_t s) {
     const uint32_t n = a.item1 - a.item0;
     if (n == 0) return hipSuccess;
     if (a.k == 0 || a.k > (uint32_t)kMaxK) return hipErrorInvalidValue;
+    if (!a.n_at && (!a.bounds || !a.bnres || a.nb == 0)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_knn_prefetch, dim3((n + kWavesPerWG - 1) / kWavesPerWG), dim3(kWave * kWavesPerWG),
                        kWavesPerWG * sizeof(WaveLds), s, a);
     return hipGetLastError();
