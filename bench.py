@@ -251,6 +251,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="time the library's default launch structure")
     ap.add_argument("--batched", type=int, default=256, help="syntheses in the batched leg (0 = skip)")
     ap.add_argument("--guided", type=int, default=3, help="runs of the guided leg (0 = skip)")
     ap.add_argument("--serve-requests", type=int, default=480, help="requests in the serving leg (0 = skip)")
@@ -274,6 +276,13 @@ def main():
     ctx = ts.Context.default()
     for _ in range(args.warmup):
         build_session(ts, ex).run()
+    # launch structure for this workload, picked on the device before the
+    # timed steps (results do not depend on it); later sessions of the shape get it
+    tuned = None
+    if args.autotune:
+        from tsgpu.autotune import autotune
+        b = ts.Session.builder().add_example(ex).seed(0).output_size(ts.Dims.square(1024))
+        tuned = autotune(b, reps=2, ctx=ctx)
     sessions = [build_session(ts, ex) for _ in range(args.steps)]
     prof = build_session(ts, ex, profile=True)
     # a step: the synthesis with its output left in device memory, then the
@@ -320,7 +329,10 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": WORKLOAD, "output_px": 1024 * 1024, "resolves_per_step": resolves,
-                   "schedule": "jacobi " + json.dumps(ts.default_schedule()), "batches_per_step": g.n_batches},
+                   "schedule": "jacobi " + json.dumps(ts.default_schedule()), "batches_per_step": g.n_batches,
+                   "launch_structure": (dict(tuned["tuning"], autotuned=True, default_ms=round(tuned["default_ms"], 3),
+                                             tuned_ms=round(tuned["ms"], 3), candidates=len(tuned["tried"]))
+                                        if tuned else dict(ts.default_tuning(), autotuned=False))},
         "latency_ms": {"p50": round(statistics.median(lat), 3), "p99": round(p99, 3)},
         "pcie_inclusive": {"value": round(1024 * 1024 * args.steps / t_pcie / 1e6, 3), "unit": "Mpix/s",
                            "ms_per_step": round(t_pcie * 1e3 / args.steps, 3),

@@ -161,8 +161,16 @@ int tsg_job_create(tsg_ctx* ctx, const tsg_params* params, uint32_t n_examples, 
                    uint32_t out_w, uint32_t out_h, tsg_job** out);
 /* Overrides the default Jacobi schedule (before tsg_job_run). */
 int tsg_job_set_schedule(tsg_job* job, const tsg_schedule* s);
-/* Overrides the default launch structure (before tsg_job_run). */
+/* Overrides the launch structure (before tsg_job_run). A new job starts
+ * with the one its context stored for its shape, else tsg_tuning_default. */
 int tsg_job_set_tuning(tsg_job* job, const tsg_tuning* t);
+int tsg_job_get_tuning(const tsg_job* job, tsg_tuning* out);
+/* Autotuner cache (no counterpart in the reference): stores t for jobs of
+ * the same shape as `like` (output size, k, m, sampled maps, guided, tiling,
+ * stages) created on ctx from now on; the search that picks t is the
+ * caller's (tsgpu.autotune times candidate structures on the device). */
+int tsg_ctx_store_tuning(tsg_ctx* ctx, const tsg_job* like, const tsg_tuning* t);
+int tsg_ctx_clear_tunings(tsg_ctx* ctx);
 /* Session::run (session.rs:37-66): random init, then every stage. */
 int tsg_job_run(tsg_job* job, tsg_progress_fn progress, void* user);
 /* GeneratedImage accessors (lib.rs:378-455, ms.rs:605-684). Any pointer may

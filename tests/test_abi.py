@@ -50,3 +50,10 @@ def test_xform_round_trip():
     assert back.output_size == tsgpu.Dims(2, 1)
     assert back.original_maps == [tsgpu.Dims(9001, 9002), tsgpu.Dims(20, 5)]
     assert data[:4] == bytes([0x01, 0x00, 0x34, 0x12])
+
+
+def test_autotune_space_names_tuning_knobs():
+    from tsgpu.autotune import SPACE
+    t = tsgpu.default_tuning()
+    for knob, values in SPACE:
+        assert knob in t and len(values) >= 2

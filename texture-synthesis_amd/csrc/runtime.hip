@@ -479,6 +479,8 @@ struct tsg_ctx {
     uint32_t offtab_n = 0;
     DBuf<float> dummy_lut;
     std::mutex mu;
+    // launch structures stored by the autotuner, by job shape (see tuning_key)
+    std::map<std::vector<uint64_t>, tsg_tuning> tunings;
 };
 
 struct tsg_job {
@@ -1534,6 +1536,25 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
     return step_finish(j);
 }
 
+// Launch structures the autotuner stores apply to jobs of the same shape:
+// output size, k, m, sampled maps, guided, tiling, stages.
+std::vector<uint64_t> tuning_key(const tsg_job* j) {
+    return {j->W, j->H, j->prm.nearest_neighbors, j->prm.random_sample_locations, j->filt.size(),
+            (uint64_t)j->guided, j->prm.tiling_mode, j->prm.backtrack_stages};
+}
+
+int check_tuning(const tsg_tuning* t) {
+    if (t->small_max > small_batch_max()) return fail(TSG_ERR_INVALID_ARGUMENT, "small_max above 16");
+    if (t->team_max == 0 || t->team_max > 8 || (t->team_max & (t->team_max - 1)))
+        return fail(TSG_ERR_INVALID_ARGUMENT, "team_max must be 1, 2, 4 or 8");
+    if (t->coop_max > t->coop_wgs * small_batch_max())
+        return fail(TSG_ERR_INVALID_ARGUMENT, "coop_max above 16 items per cooperative workgroup");
+    if (t->flow && (t->flow_team == 0 || t->flow_team > 8 || (t->flow_team & (t->flow_team - 1))))
+        return fail(TSG_ERR_INVALID_ARGUMENT, "flow_team must be 1, 2, 4 or 8");
+    if (t->ahead > 2) return fail(TSG_ERR_INVALID_ARGUMENT, "ahead must be 0, 1 or 2");
+    return TSG_OK;
+}
+
 // ---- multi-job engine ----
 
 size_t job_footprint(const tsg_job* j) {
@@ -2012,6 +2033,11 @@ int tsg_job_create(tsg_ctx* ctx, const tsg_params* prm, uint32_t n_examples, con
     if ((rc = init_state(j.get()))) return rc;
     j->state_ready = true;
     HIP_TRY(hipStreamSynchronize(s));
+    {
+        // a launch structure the autotuner stored for this shape (ctx->mu is held)
+        auto it = ctx->tunings.find(tuning_key(j.get()));
+        if (it != ctx->tunings.end()) j->tune = it->second;
+    }
     *out = j.release();
     return TSG_OK;
 }
@@ -2025,7 +2051,7 @@ void tsg_tuning_default(tsg_tuning* t) {
     t->fuse_max = 0;
     t->team_max = 8;
     t->team_waves = 4096;
-    t->prefetch_max = 2048;
+    t->prefetch_max = 1024;  // autotuned: 26.3 ms against 27.2 ms at 2048
     // 2 (a set per stage, every neighbour list ahead) measured 27.2 ms against
     // 26.9 ms: the second stream's kernels wait for free slots behind the
     // latency-bound launches instead of filling them
@@ -2039,14 +2065,32 @@ void tsg_tuning_default(tsg_tuning* t) {
 
 int tsg_job_set_tuning(tsg_job* job, const tsg_tuning* t) {
     if (!job || !t) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
-    if (t->small_max > small_batch_max()) return fail(TSG_ERR_INVALID_ARGUMENT, "small_max above 16");
-    if (t->team_max == 0 || t->team_max > 8 || (t->team_max & (t->team_max - 1)))
-        return fail(TSG_ERR_INVALID_ARGUMENT, "team_max must be 1, 2, 4 or 8");
-    if (t->coop_max > t->coop_wgs * small_batch_max())
-        return fail(TSG_ERR_INVALID_ARGUMENT, "coop_max above 16 items per cooperative workgroup");
-    if (t->flow && (t->flow_team == 0 || t->flow_team > 8 || (t->flow_team & (t->flow_team - 1))))
-        return fail(TSG_ERR_INVALID_ARGUMENT, "flow_team must be 1, 2, 4 or 8");
+    int rc = check_tuning(t);
+    if (rc) return rc;
     job->tune = *t;
+    return TSG_OK;
+}
+
+int tsg_job_get_tuning(const tsg_job* job, tsg_tuning* out) {
+    if (!job || !out) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    *out = job->tune;
+    return TSG_OK;
+}
+
+int tsg_ctx_store_tuning(tsg_ctx* ctx, const tsg_job* like, const tsg_tuning* t) {
+    if (!ctx || !like || !t) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    if (like->ctx != ctx) return fail(TSG_ERR_INVALID_ARGUMENT, "the job belongs to another context");
+    int rc = check_tuning(t);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->tunings[tuning_key(like)] = *t;
+    return TSG_OK;
+}
+
+int tsg_ctx_clear_tunings(tsg_ctx* ctx) {
+    if (!ctx) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->tunings.clear();
     return TSG_OK;
 }
 

@@ -130,6 +130,9 @@ def lib():
     L.tsg_job_set_schedule.argtypes = [vp, P(Schedule)]
     L.tsg_tuning_default.argtypes = [P(Tuning)]
     L.tsg_job_set_tuning.argtypes = [vp, P(Tuning)]
+    L.tsg_job_get_tuning.argtypes = [vp, P(Tuning)]
+    L.tsg_ctx_store_tuning.argtypes = [vp, vp, P(Tuning)]
+    L.tsg_ctx_clear_tunings.argtypes = [vp]
     L.tsg_job_run.argtypes = [vp, PROGRESS_FN, vp]
     L.tsg_job_download.argtypes = [vp, vp, vp, vp, vp, vp]
     L.tsg_job_download_packed.argtypes = [vp, vp, vp, vp]
@@ -211,6 +214,19 @@ class Context:
         if cls._default is None:
             cls._default = Context(0)
         return cls._default
+
+    def store_tuning(self, like: "Session", **tuning) -> None:
+        """Launch structure for jobs shaped like `like` created on this context
+        from now on (keys of default_tuning(); the rest stay default)."""
+        t = default_tuning()
+        unknown = set(tuning) - set(t)
+        if unknown:
+            raise TypeError(f"unknown tuning keys {sorted(unknown)}")
+        t.update(tuning)
+        _check(lib().tsg_ctx_store_tuning(self._h, like._job, C.byref(Tuning(**t))))
+
+    def clear_tunings(self) -> None:
+        _check(lib().tsg_ctx_clear_tunings(self._h))
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
@@ -392,6 +408,12 @@ class Session:
         s = Schedule(px_per_batch, redo_shift, new_shift, max_batch, first_new_shift)
         _check(lib().tsg_job_set_schedule(self._job, C.byref(s)))
         return self
+
+    def tuning(self) -> dict:
+        """The launch structure this job will run with."""
+        t = Tuning()
+        _check(lib().tsg_job_get_tuning(self._job, C.byref(t)))
+        return {n: getattr(t, n) for n, _ in Tuning._fields_}
 
     def set_tuning(self, **kw) -> "Session":
         """Launch structure (results do not depend on it); keys of default_tuning()."""
