@@ -85,6 +85,7 @@ struct WaveLds {
     uint32_t t_kf;
     float t_B;
     uint32_t t_Bi;
+    uint32_t t_xy, t_ms;  // the member's best candidate's coordinate and map word
     unsigned long long t_bound;
 };
 
@@ -598,6 +599,7 @@ __device__ __forceinline__ float term_of(const WaveLds* L, const float* lut, con
 struct Best {
     float B;      // lowest complete cost so far (FLT_MAX: none)
     uint32_t Bi;  // its candidate index (kNone: none)
+    uint32_t xy, ms;  // its coordinate and map word (bit 31: mirrored pattern)
 };
 
 // Evaluates the n_act candidates in L->act_* with exact pruning. Terms of a
@@ -606,17 +608,19 @@ struct Best {
 // reference's sequential f32 sum). A candidate whose partial sum already
 // exceeds the best complete cost (or equals it with a later index) can
 // never win and is dropped: partial sums of non-negative terms only grow.
-// The team's bound into `best` when it is lexicographically smaller; read by
-// one lane so the wave sees one value.
-__device__ __forceinline__ void take_team_bound(const unsigned long long* tb, Best& best) {
+// The pruning bound: the lexicographically smaller of the member's own best
+// and the team's (read by one lane so the wave sees one value).
+__device__ __forceinline__ void prune_bound(const unsigned long long* tb, const Best& best, float& PB, uint32_t& PBi) {
+    PB = best.B;
+    PBi = best.Bi;
     if (!tb) return;
     const unsigned long long v = __hip_atomic_load(tb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     const uint32_t vi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
     const uint32_t vb = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
     const float B = __uint_as_float(vb);
-    if (vi != kNone && (best.Bi == kNone || B < best.B || (B == best.B && vi < best.Bi))) {
-        best.B = B;
-        best.Bi = vi;
+    if (vi != kNone && (PBi == kNone || B < PB || (B == PB && vi < PBi))) {
+        PB = B;
+        PBi = vi;
     }
 }
 
@@ -625,12 +629,14 @@ __device__ __forceinline__ void eval_group(const MapDesc* maps, WaveLds* L, cons
                            uint32_t kf, uint32_t n_act, Best& best, unsigned long long* tb) {
     if (lane < (int)n_act) L->act_sum[lane] = 0.0f;
     wave_sync();
-    take_team_bound(tb, best);
+    float PB;
+    uint32_t PBi;
+    prune_bound(tb, best, PB, PBi);
     uint32_t j0 = 0;
     while (n_act > 0) {
         // chunk of 2^sh neighbours with n_act << sh <= kTermCap; while a bound
         // exists the first chunk is kept at 8 neighbours so pruning starts early
-        uint32_t sh = (j0 == 0 && best.Bi != kNone) ? 3 : 6;
+        uint32_t sh = (j0 == 0 && PBi != kNone) ? 3 : 6;
         while (sh > 0 && (n_act << sh) > (uint32_t)kTermCap) --sh;
         const uint32_t jc = 1u << sh;
         const uint32_t jn = min(jc, kf - j0);
@@ -682,18 +688,24 @@ __device__ __forceinline__ void eval_group(const MapDesc* maps, WaveLds* L, cons
         j0 += jn;
         if (j0 >= kf) {
             float key = act ? s : INFINITY;
-            uint32_t ki = myidx;
+            uint32_t ki = myidx, kxy = xy, kms = ms;
             for (int off = kWave / 2; off > 0; off >>= 1) {
                 float ok = __shfl_xor(key, off);
                 uint32_t oi = (uint32_t)__shfl_xor((int)ki, off);
+                uint32_t oxy = (uint32_t)__shfl_xor((int)kxy, off);
+                uint32_t oms = (uint32_t)__shfl_xor((int)kms, off);
                 if (ok < key || (ok == key && oi < ki)) {
                     key = ok;
                     ki = oi;
+                    kxy = oxy;
+                    kms = oms;
                 }
             }
             if (key < best.B || (key == best.B && best.Bi != kNone && ki < best.Bi)) {
                 best.B = key;
                 best.Bi = ki;
+                best.xy = kxy;
+                best.ms = kms;
             }
             // publish: another member's complete cost bounds this one's candidates
             if (tb && lane == 0 && best.Bi != kNone)
@@ -701,8 +713,8 @@ __device__ __forceinline__ void eval_group(const MapDesc* maps, WaveLds* L, cons
                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             break;
         }
-        take_team_bound(tb, best);
-        const bool keep = act && !(s > best.B || (s == best.B && (best.Bi == kNone || myidx > best.Bi)));
+        prune_bound(tb, best, PB, PBi);
+        const bool keep = act && !(s > PB || (s == PB && (PBi == kNone || myidx > PBi)));
         const uint64_t km = __ballot(keep);
         const uint32_t pos = (uint32_t)__popcll(km & lanemask_lt(lane));
         wave_sync();
@@ -767,19 +779,22 @@ __device__ __forceinline__ uint32_t item_knn(const BatchArgs& a, uint32_t item, 
 }
 
 // Leader: no resolved neighbour, resolve_at_random with the stage seed
-// (ms.rs:1003-1009).
-__device__ __forceinline__ void item_at_random(const BatchArgs& a, const MapDesc* maps, uint32_t item, uint32_t rel,
-                                               uint32_t flat, int lane) {
-    if (lane != 0) return;
+// (ms.rs:1003-1009). Returns the result (lane 0), also written to
+// result[rel] unless a.result is null.
+__device__ __forceinline__ uint4 item_at_random(const BatchArgs& a, const MapDesc* maps, uint32_t item, uint32_t rel,
+                                                uint32_t flat, int lane) {
+    if (lane != 0) return make_uint4(0, 0, 0, 0);
     Pcg32 g0 = Pcg32::seed_from_u64(a.stage_seed);
     uint32_t rm = (uint32_t)range_u64(g0, 0, a.n_maps);
     Pcg32 g1 = Pcg32::seed_from_u64(a.stage_seed);
     uint32_t rx = range_u32(g1, 0, maps[rm].w);
     Pcg32 g2 = Pcg32::seed_from_u64(a.stage_seed);
     uint32_t ry = range_u32(g2, 0, maps[rm].h);
-    a.result[rel] = make_uint4(rx | (ry << 16), rm | (rm << 16) | kFallbackBit, flat, __float_as_uint(0.0f));
+    const uint4 r = make_uint4(rx | (ry << 16), rm | (rm << 16) | kFallbackBit, flat, __float_as_uint(0.0f));
+    if (a.result) a.result[rel] = r;
     if (a.item_score) a.item_score[item] = -1.0f;
     if (a.dbg_best) a.dbg_best[rel] = 0xffffffffu;
+    return r;
 }
 
 // Every member: neighbour state (own pattern, distances, weights) into L and
@@ -969,10 +984,13 @@ __device__ __forceinline__ void item_exact(const BatchArgs& a, const MapDesc* ma
     }
 }
 
-// Leader: the winning candidate into result[rel].
-__device__ __forceinline__ void item_winner(const BatchArgs& a, const MapDesc* maps, uint32_t item, uint32_t rel,
-                                            uint32_t n_coh, const WaveLds* L, float best, uint32_t bidx, int lane) {
-    if (lane != 0) return;
+// Leader: the winning candidate. Returns the result (lane 0), also written
+// to result[rel] unless a.result is null. xm: the winner's coordinate and
+// map word when the evaluation carried them (else they are read again).
+__device__ __forceinline__ uint4 item_winner(const BatchArgs& a, const MapDesc* maps, uint32_t item, uint32_t rel,
+                                             uint32_t n_coh, const WaveLds* L, float best, uint32_t bidx, int lane,
+                                             const uint2* xm = nullptr) {
+    if (lane != 0) return make_uint4(0, 0, 0, 0);
     uint32_t cxy, cmap, patch;
     if (a.ext_cands) {
         uint4 e = a.ext_cands[(size_t)rel * a.ext_cmax + bidx];
@@ -985,14 +1003,21 @@ __device__ __forceinline__ void item_winner(const BatchArgs& a, const MapDesc* m
         cmap = e.y;
         patch = e.z;
     } else {
-        uint2 r = a.rcands[(size_t)(item - a.rc_base) * a.m + (bidx - n_coh)];
-        cxy = r.x;
-        cmap = r.y;
+        if (xm) {
+            cxy = xm->x;
+            cmap = xm->y & 0x7fffffffu;
+        } else {
+            const uint2 rc = a.rcands[(size_t)(item - a.rc_base) * a.m + (bidx - n_coh)];
+            cxy = rc.x;
+            cmap = rc.y;
+        }
         patch = (cxy >> 16) * maps[cmap].w + (cxy & 0xffffu);
     }
-    a.result[rel] = make_uint4(cxy, cmap | (cmap << 16), patch, __float_as_uint(best));
+    const uint4 r = make_uint4(cxy, cmap | (cmap << 16), patch, __float_as_uint(best));
+    if (a.result) a.result[rel] = r;
     if (a.item_score) a.item_score[item] = best;
     if (a.dbg_best) a.dbg_best[rel] = bidx;
+    return r;
 }
 
 // Resolves work item `item` of the batch starting at item0 against the state
@@ -1031,10 +1056,12 @@ __device__ __forceinline__ void flow_wait(const FlowArgs& f, const BatchArgs& a,
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
+// Returns the result in the leader's lane 0 (zero elsewhere).
 template <bool GUIDED>
-__device__ void resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t item, uint32_t item0, uint32_t n_res,
-                             WaveLds* Lt, int member, int T, bool valid, const float* lut, const float* glut,
-                             int lane, const FlowArgs* flow = nullptr) {
+__device__ uint4 resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t item, uint32_t item0, uint32_t n_res,
+                              WaveLds* Lt, int member, int T, bool valid, const float* lut, const float* glut,
+                              int lane, const FlowArgs* flow = nullptr) {
+    uint4 res = make_uint4(0, 0, 0, 0);
     WaveLds* L = Lt + member;
     const int slot = lane % kLutCopies;
     const uint32_t rel = item - item0;
@@ -1094,7 +1121,7 @@ __device__ void resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t i
         wave_sync();
         if (lead) phase_clock(a, item, 1, lane);
         if (kf == 0) {
-            if (lead) item_at_random(a, maps, item, rel, flat, lane);
+            if (lead) res = item_at_random(a, maps, item, rel, flat, lane);
             done = true;
         }
     }
@@ -1123,7 +1150,7 @@ __device__ void resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t i
             const uint32_t nv = ncs + nrs;
             const uint32_t upos = (uint32_t)__popcll(uniq & lanemask_lt(lane));
             const bool mine_u = ((uniq >> lane) & 1ull) && upos >= c0 && upos < c0 + ncs;
-            Best bst{FLT_MAX, kNone};
+            Best bst{FLT_MAX, kNone, 0u, 0u};
             uint32_t g_lo = 0;
             bool first = true;
             while (g_lo < nv) {
@@ -1170,12 +1197,14 @@ __device__ void resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t i
             if (lane == 0) {
                 L->t_B = bst.B;
                 L->t_Bi = bst.Bi;
+                L->t_xy = bst.xy;
+                L->t_ms = bst.ms;
             }
         } else if (lead) {
             float best = FLT_MAX;
             uint32_t bidx = 0;
             item_exact<GUIDED>(a, maps, item, rel, kf, n_coh, n_c, L, lut, glut, slot, lane, best, bidx);
-            item_winner(a, maps, item, rel, n_coh, L, best, bidx, lane);
+            res = item_winner(a, maps, item, rel, n_coh, L, best, bidx, lane);
         }
     }
     if (T > 1) __syncthreads();
@@ -1184,21 +1213,25 @@ __device__ void resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t i
     if (!done && !exact && lead) {
         float best = FLT_MAX;
         uint32_t bidx = kNone;
+        uint2 bxm = make_uint2(0, 0);
         for (int q = 0; q < T; ++q) {
             const float B = Lt[q].t_B;
             const uint32_t Bi = Lt[q].t_Bi;
             if (Bi != kNone && (bidx == kNone || B < best || (B == best && Bi < bidx))) {
                 best = B;
                 bidx = Bi;
+                bxm = make_uint2(Lt[q].t_xy, Lt[q].t_ms);
             }
         }
-        if (bidx == kNone) {
+        const bool have = bidx != kNone;
+        if (!have) {
             best = FLT_MAX;
             bidx = 0;
         }
-        item_winner(a, maps, item, rel, n_coh, L, best, bidx, lane);
+        res = item_winner(a, maps, item, rel, n_coh, L, best, bidx, lane, have ? &bxm : nullptr);
     }
     if (valid && lead) phase_clock(a, item, 6, lane);
+    return res;
 }
 
 
@@ -1366,8 +1399,8 @@ __global__ __launch_bounds__(kWave* kSmallWaves) void k_resolve_coop(BatchArgs a
 
 // A dependency-ordered run of new-pixel items (FlowArgs): one item per
 // workgroup at a time, the workgroup's waves as its team. Thread 0 is the
-// team leader's lane 0: it writes the item's result into the workgroup's
-// slot, commits the pixel and publishes it with a device-scope release.
+// team leader's lane 0: it holds the item's result, commits the pixel and
+// publishes it with a device-scope release. a.result is not used.
 template <bool GUIDED>
 __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve_flow(BatchArgs a, FlowArgs f) {
     const uint64_t t_start = a.dbg_launch ? wall_clock64() : 0;
@@ -1387,12 +1420,19 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve_flow(BatchArgs a
         const uint32_t idx = s_idx;
         if (idx >= f.n) break;
         const uint32_t item = a.item0 + idx;
-        const uint32_t slot0 = item - blockIdx.x;  // result[item - slot0] is this workgroup's slot
-        resolve_team<GUIDED>(a, maps, item, slot0, 0u, wls, wave, T, true, lut, glut, lane, &f);
+        const uint32_t p = threadIdx.x == 0 ? a.work[item] : 0u;
+        const uint4 r = resolve_team<GUIDED>(a, maps, item, item, 0u, wls, wave, T, true, lut, glut, lane, &f);
         if (threadIdx.x == 0) {
-            commit_one(a.W, a.tiles_x, maps, a.work, a.result, item, slot0, 0u, a.state_w, a.mask_w, a.score_w,
-                       a.gmask);
-            __hip_atomic_store(f.flag + a.work[item], f.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            // Generator::update (ms.rs:334-377): the state record, published
+            // at once; the mask bit and the score are read only after the run
+            const MapDesc& md = maps[r.y & 0xffffu];
+            const uint32_t color = md.px[(r.x >> 16) * md.w + (r.x & 0xffffu)];
+            a.state_w[p] = make_uint4(color, r.x, r.y & ~kFallbackBit, r.z);
+            __hip_atomic_store(f.flag + p, f.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t px = p % a.W, py = p / a.W;
+            atomicOr((unsigned long long*)&a.mask_w[(py >> 3) * a.tiles_x + (px >> 3)],
+                     1ull << (((py & 7u) << 3) | (px & 7u)));
+            a.score_w[p] = __uint_as_float(r.w);
             phase_clock(a, item, 10, 0);
         }
     }

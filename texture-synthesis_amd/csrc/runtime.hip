@@ -1414,7 +1414,6 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
             HIP_TRY(hipMemsetAsync(j->flow_ticket.p, 0, 4, s));
             j->flow_tickets = 0;
         }
-        HIP_TRY(j->result.reserve(flow_wgs));
     }
     int last_pct = 0;
     std::vector<uint8_t> snap;
@@ -1485,7 +1484,11 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
                 const uint32_t g = std::min(flow_wgs, n);
                 const FlowArgs f{j->flow_ticket.p, j->flow_tickets, n, j->aset(st.t).dep.p,
                                  j->flow_flag.p, j->flow_epoch, kFlowSpin, j->overflow.p};
-                HIP_TRY(timed_launch(j, K_RUN, n, [&] { return launch_resolve_flow(a, f, g, flow_team, j->guided, s); }));
+                HIP_TRY(timed_launch(j, K_RUN, n, [&] {
+                    BatchArgs fa = a;  // results stay in registers
+                    fa.result = nullptr;
+                    return launch_resolve_flow(fa, f, g, flow_team, j->guided, s);
+                }));
                 j->flow_tickets += n + g;
                 while (st.i < flow_end) {
                     const uint64_t b0 = st.i, b1 = batch_end(j, b0, st.redo, st.S_eff);
