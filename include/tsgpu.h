@@ -182,6 +182,19 @@ int tsg_job_run(tsg_job* job, tsg_progress_fn progress, void* user);
  * id-map id << 16, patch id}, plus the resolved list and its scores (each
  * pointer may be NULL). */
 int tsg_job_download_packed(tsg_job* job, uint32_t* state, uint32_t* resolved_flat, float* resolved_score);
+/* GeneratedImage::save_debug's maps of a finished job (Generator::get_id_maps
+ * and get_uncertainty_map, ms.rs:603-653), RGBA8 w*h*4 each, any may be NULL:
+ * patch ids and map ids coloured by seeded gen_range(0..255) draws, and the
+ * uncertainty [n, 255 - n, 0, 255] with n = (min(score, 1) * 255) as u8 of
+ * every resolved pixel (others stay 0). */
+int tsg_job_debug_maps(tsg_job* job, uint8_t* patch_ids, uint8_t* map_ids, uint8_t* uncertainty);
+/* CoordinateTransform::apply (lib.rs:178-210) on the device: out (w*h*4
+ * RGBA8) pixel i = sources[map][y][x] for coord triple i = (x, y, map), the
+ * .xform body. Sources at the transform's original sizes (resizing them is
+ * the caller's, as load_image does). A coordinate outside its source (the
+ * reference panics) returns TSG_ERR_INVALID_ARGUMENT. */
+int tsg_xform_apply(tsg_ctx* ctx, const uint32_t* coord, uint32_t w, uint32_t h, uint32_t n_sources,
+                    const tsg_image* sources, uint8_t* out);
 int tsg_job_download(tsg_job* job, uint8_t* rgba, uint32_t* coord, uint32_t* ids, uint32_t* resolved_flat,
                      float* resolved_score);
 /* Per stage (index 0 = p_stage P): sum of the winning costs of

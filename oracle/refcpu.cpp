@@ -98,6 +98,18 @@ inline uint32_t range_u32(Pcg32& g, uint32_t lo, uint32_t hi) {
         if ((uint32_t)m <= zone) return lo + (uint32_t)(m >> 32);
     }
 }
+/* rand 0.8.5 UniformInt::<u8 / u16>::sample_single: u32 draws; for types of
+ * at most 16 bits the zone is u32::MAX - (u32::MAX - range + 1) % range. */
+inline uint32_t range_u8(Pcg32& g, uint32_t lo, uint32_t hi) {
+    const uint32_t range = hi - lo;
+    const uint32_t unsigned_max = 0xffffffffu;
+    const uint32_t zone = unsigned_max - (unsigned_max - range + 1u) % range;
+    for (;;) {
+        uint32_t v = g.next_u32();
+        uint64_t m = (uint64_t)v * range;
+        if ((uint32_t)m <= zone) return lo + (uint32_t)(m >> 32);
+    }
+}
 /* same for u64 / usize (x86-64): draws next_u64 */
 inline uint64_t range_u64(Pcg32& g, uint64_t lo, uint64_t hi) {
     uint64_t range = hi - lo;
@@ -1108,6 +1120,48 @@ uint64_t orc_job_resolved(orc_job* j, uint32_t* flat, float* score, uint64_t cap
     return j->resolved.size();
 }
 uint64_t orc_job_locked(orc_job* j) { return j->locked; }
+
+/* get_id_maps (ms.rs:603-630): Rgba([gen_range(0..255) of three seeded
+ * streams, 255]); the seeds are u32 expressions of the id (wrapping, as the
+ * release build computes them) widened to u64, except map * 200 in u64.
+ * get_uncertainty_map (ms.rs:633-653): for each (pixel, score) of the
+ * resolved list, n = (score.min(1.0) * 255.0) as u8 (saturating). */
+void orc_job_debug_maps(orc_job* j, uint8_t* patch_ids, uint8_t* map_ids, uint8_t* uncertainty) {
+    const size_t N = (size_t)j->W * j->H;
+    auto draw = [](uint64_t seed) {
+        Pcg32 g = Pcg32::seed_from_u64(seed);
+        return (uint8_t)range_u8(g, 0, 255);
+    };
+    for (size_t i = 0; i < N; ++i) {
+        const uint32_t patch = j->pid[i], map = j->pm[i];
+        if (patch_ids) {
+            uint8_t* o = patch_ids + 4 * i;
+            o[0] = draw(patch);
+            o[1] = draw((uint32_t)(patch * 5u + 21u));
+            o[2] = draw((uint32_t)(patch / 4u + 12u));
+            o[3] = 255;
+        }
+        if (map_ids) {
+            uint8_t* o = map_ids + 4 * i;
+            o[0] = draw((uint64_t)map * 200u);
+            o[1] = draw((uint32_t)(map * 5u + 341u));
+            o[2] = draw((uint32_t)(map * 1200u - 35412u));
+            o[3] = 255;
+        }
+    }
+    if (uncertainty) {
+        memset(uncertainty, 0, N * 4);
+        for (size_t q = 0; q < j->resolved.size(); ++q) {
+            const float v = std::fmin(j->rscore[q], 1.0f) * 255.0f;
+            const uint8_t n = !(v == v) ? 0 : v <= 0.0f ? 0 : v >= 255.0f ? 255 : (uint8_t)v;
+            uint8_t* o = uncertainty + 4 * (size_t)j->resolved[q];
+            o[0] = n;
+            o[1] = (uint8_t)(255 - n);
+            o[2] = 0;
+            o[3] = 255;
+        }
+    }
+}
 
 uint32_t orc_job_stage_stats(orc_job* j, double* sums, uint64_t* counts, uint64_t* items) {
     for (size_t i = 0; i < j->ssum.size(); ++i) {

@@ -95,6 +95,9 @@ int orc_job_run_until(orc_job* j, const orc_sched* s, uint32_t n_threads, uint32
 void orc_job_get(orc_job* j, uint8_t* rgba, uint32_t* coord, uint32_t* ids);
 uint64_t orc_job_resolved(orc_job* j, uint32_t* flat, float* score, uint64_t cap);
 uint64_t orc_job_locked(orc_job* j);
+/* Generator::get_id_maps / get_uncertainty_map (ms.rs:603-653) of the
+ * current state: RGBA8 w*h*4 each, any may be NULL. */
+void orc_job_debug_maps(orc_job* j, uint8_t* patch_ids, uint8_t* map_ids, uint8_t* uncertainty);
 /* per-stage: sum of best scores, count of pattern-matched items, items done. */
 uint32_t orc_job_stage_stats(orc_job* j, double* sums, uint64_t* counts, uint64_t* items);
 /* resolved mask (1 byte per px) */

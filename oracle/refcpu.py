@@ -68,6 +68,8 @@ def lib():
         L.orc_job_get.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_job_resolved.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]
         L.orc_job_resolved.restype = C.c_uint64
+        L.orc_job_debug_maps.argtypes = [C.c_void_p] * 4
+        L.orc_job_debug_maps.restype = None
         L.orc_job_locked.argtypes = [C.c_void_p]
         L.orc_job_locked.restype = C.c_uint64
         L.orc_job_stage_stats.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -266,6 +268,12 @@ class Job:
         m = np.zeros((self.h, self.w), np.uint8)
         lib().orc_job_mask(self._h, _ptr(m))
         return m
+
+    def debug_maps(self):
+        """get_id_maps and get_uncertainty_map: (patch ids, map ids, uncertainty) RGBA8."""
+        maps = [np.zeros((self.h, self.w, 4), np.uint8) for _ in range(3)]
+        lib().orc_job_debug_maps(self._h, *[_ptr(m) for m in maps])
+        return tuple(maps)
 
     def stage_stats(self):
         P = self.params.backtrack_stages

@@ -77,6 +77,18 @@ TSG_HD uint32_t range_u32(Pcg32& g, uint32_t lo, uint32_t hi) {
     }
 }
 
+// gen_range(lo..hi) for u8 and u16: 32-bit draws and a widening multiply,
+// rejection zone u32::MAX - (2^32 - range) % range (types of at most 16 bits).
+TSG_HD uint32_t range_small(Pcg32& g, uint32_t lo, uint32_t hi) {
+    const uint32_t range = hi - lo;
+    const uint32_t zone = 0xffffffffu - (uint32_t)((0x100000000ull - range) % range);
+    for (;;) {
+        const uint32_t v = g.next_u32();
+        const uint64_t m = (uint64_t)v * range;
+        if ((uint32_t)m <= zone) return lo + (uint32_t)(m >> 32);
+    }
+}
+
 // gen_range(lo..hi) for u64 / usize: same method on next_u64 draws.
 TSG_HD uint64_t range_u64(Pcg32& g, uint64_t lo, uint64_t hi) {
     uint64_t range = hi - lo;

@@ -161,6 +161,22 @@ struct UnpackArgs {
 };
 hipError_t launch_unpack(const UnpackArgs& a, hipStream_t s);
 
+// Debug maps of a finished job (GeneratedImage::save_debug, ms.rs:603-653):
+// patch and map ids coloured by seeded draws, and the uncertainty of each
+// resolved pixel (out_unc is all zero for the others). Outputs are RGBA8
+// packed in u32; null outputs are skipped.
+hipError_t launch_id_maps(const uint4* state, uint32_t n, uint32_t* patch_rgba, uint32_t* map_rgba, hipStream_t s);
+hipError_t launch_uncertainty(const uint32_t* rlist, const float* score, uint32_t n_res, uint32_t* out_unc,
+                              hipStream_t s);
+// CoordinateTransform::apply (lib.rs:178-210): out[i] = source map's pixel at
+// (x, y) of triple i. A coordinate outside its map sets *bad.
+struct XformSource {
+    const uint32_t* px;
+    uint32_t w, h;
+};
+hipError_t launch_xform_apply(const uint32_t* coord, uint32_t n, const XformSource* srcs, uint32_t n_src,
+                              uint32_t* out, uint32_t* bad, hipStream_t s);
+
 // per-stage statistics: sum and count of non-negative item scores
 struct StatsArgs {
     const float* item_score;

@@ -1540,6 +1540,48 @@ __global__ void k_unpack(UnpackArgs a) {
     if (a.rscore && i < a.n_res) a.rscore[i] = a.score[a.rlist[i]];
 }
 
+// get_id_maps (ms.rs:603-630): each channel is gen_range(0..255) of its own
+// seeded stream; seeds are u32 arithmetic on the ids (wrapping) widened to u64.
+__device__ __forceinline__ uint32_t id_colour(uint64_t s0, uint64_t s1, uint64_t s2) {
+    Pcg32 g0 = Pcg32::seed_from_u64(s0), g1 = Pcg32::seed_from_u64(s1), g2 = Pcg32::seed_from_u64(s2);
+    const uint32_t r = range_small(g0, 0, 255), g = range_small(g1, 0, 255), b = range_small(g2, 0, 255);
+    return r | (g << 8) | (b << 16) | 0xff000000u;
+}
+
+__global__ void k_id_maps(const uint4* state, uint32_t n, uint32_t* patch_rgba, uint32_t* map_rgba) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint4 st = state[i];
+    const uint32_t patch = st.w, map = st.z >> 16;
+    if (patch_rgba) patch_rgba[i] = id_colour(patch, patch * 5u + 21u, patch / 4u + 12u);
+    if (map_rgba) map_rgba[i] = id_colour((uint64_t)map * 200u, map * 5u + 341u, map * 1200u - 35412u);
+}
+
+// get_uncertainty_map (ms.rs:633-653): n = (min(score, 1) * 255) as u8,
+// colour [n, 255 - n, 0, 255], for every pixel of the resolved list.
+__global__ void k_uncertainty(const uint32_t* rlist, const float* score, uint32_t n_res, uint32_t* out) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_res) return;
+    const uint32_t p = rlist[q];
+    const float v = fminf(score[p], 1.0f) * 255.0f;
+    // Rust's float-to-u8 cast saturates (NaN to 0)
+    const uint32_t c = v == v ? (uint32_t)fminf(fmaxf(v, 0.0f), 255.0f) : 0u;
+    out[p] = c | ((255u - c) << 8) | 0xff000000u;
+}
+
+__global__ void k_xform_apply(const uint32_t* coord, uint32_t n, const XformSource* srcs, uint32_t n_src,
+                              uint32_t* out, uint32_t* bad) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t x = coord[3 * i], y = coord[3 * i + 1], m = coord[3 * i + 2];
+    if (m >= n_src || x >= srcs[m].w || y >= srcs[m].h) {
+        out[i] = 0u;
+        atomicOr(bad, 1u);
+        return;
+    }
+    out[i] = srcs[m].px[(size_t)y * srcs[m].w + x];
+}
+
 __global__ void k_rank(const uint32_t* rlist, uint32_t lo, uint32_t hi, uint32_t* rank) {
     const uint32_t q = lo + blockIdx.x * blockDim.x + threadIdx.x;
     if (q < hi) rank[rlist[q]] = q;
@@ -1738,6 +1780,26 @@ hipError_t launch_unpack(const UnpackArgs& a, hipStream_t s) {
     const uint32_t n = a.n > a.n_res ? a.n : a.n_res;
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_unpack, dim3((n + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_id_maps(const uint4* state, uint32_t n, uint32_t* patch_rgba, uint32_t* map_rgba, hipStream_t s) {
+    if (n == 0 || (!patch_rgba && !map_rgba)) return hipSuccess;
+    hipLaunchKernelGGL(k_id_maps, dim3((n + 255) / 256), dim3(256), 0, s, state, n, patch_rgba, map_rgba);
+    return hipGetLastError();
+}
+
+hipError_t launch_uncertainty(const uint32_t* rlist, const float* score, uint32_t n_res, uint32_t* out_unc,
+                              hipStream_t s) {
+    if (n_res == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_uncertainty, dim3((n_res + 255) / 256), dim3(256), 0, s, rlist, score, n_res, out_unc);
+    return hipGetLastError();
+}
+
+hipError_t launch_xform_apply(const uint32_t* coord, uint32_t n, const XformSource* srcs, uint32_t n_src,
+                              uint32_t* out, uint32_t* bad, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_xform_apply, dim3((n + 255) / 256), dim3(256), 0, s, coord, n, srcs, n_src, out, bad);
     return hipGetLastError();
 }
 

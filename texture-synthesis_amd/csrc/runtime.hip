@@ -2142,6 +2142,69 @@ int tsg_job_download_packed(tsg_job* job, uint32_t* state, uint32_t* resolved_fl
     return TSG_OK;
 }
 
+int tsg_job_debug_maps(tsg_job* job, uint8_t* patch_ids, uint8_t* map_ids, uint8_t* uncertainty) {
+    if (!job) return fail(TSG_ERR_INVALID_ARGUMENT, "null job");
+    HIP_TRY(hipSetDevice(job->ctx->device));
+    std::lock_guard<std::mutex> lk(job->ctx->mu);
+    if (job->step.active || !job->state_ready) return fail(TSG_ERR_INVALID_ARGUMENT, "the job has no finished state");
+    hipStream_t s = job->ctx->side;
+    const size_t N = job->N;
+    DBuf<uint32_t> out;
+    HIP_TRY(out.alloc(3 * N));
+    uint32_t* dp = patch_ids ? out.p : nullptr;
+    uint32_t* dm = map_ids ? out.p + N : nullptr;
+    HIP_TRY(launch_id_maps(job->state.p, (uint32_t)N, dp, dm, s));
+    if (uncertainty) {
+        HIP_TRY(hipMemsetAsync(out.p + 2 * N, 0, N * 4, s));
+        HIP_TRY(launch_uncertainty(job->rlist.p, job->score.p, (uint32_t)job->n_res, out.p + 2 * N, s));
+    }
+    if (patch_ids) HIP_TRY(hipMemcpyAsync(patch_ids, out.p, N * 4, hipMemcpyDeviceToHost, s));
+    if (map_ids) HIP_TRY(hipMemcpyAsync(map_ids, out.p + N, N * 4, hipMemcpyDeviceToHost, s));
+    if (uncertainty) HIP_TRY(hipMemcpyAsync(uncertainty, out.p + 2 * N, N * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return TSG_OK;
+}
+
+int tsg_xform_apply(tsg_ctx* ctx, const uint32_t* coord, uint32_t w, uint32_t h, uint32_t n_sources,
+                    const tsg_image* sources, uint8_t* out) {
+    if (!ctx || !coord || !out || (n_sources && !sources)) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    if (w == 0 || h == 0) return fail(TSG_ERR_INVALID_ARGUMENT, "empty output");
+    for (uint32_t m = 0; m < n_sources; ++m)
+        if (!valid_image(&sources[m])) return fail(TSG_ERR_INVALID_ARGUMENT, "invalid source image");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    hipStream_t s = ctx->side;
+    const size_t n = (size_t)w * h;
+    size_t px = 0;
+    for (uint32_t m = 0; m < n_sources; ++m) px += (size_t)sources[m].w * sources[m].h;
+    DBuf<uint32_t> dcoord, dpx, dout, bad;
+    DBuf<XformSource> dsrc;
+    HIP_TRY(dcoord.alloc(3 * n));
+    HIP_TRY(dpx.alloc(px));
+    HIP_TRY(dout.alloc(n));
+    HIP_TRY(bad.alloc(1));
+    HIP_TRY(dsrc.alloc(std::max(1u, n_sources)));
+    std::vector<XformSource> hs(n_sources);
+    size_t off = 0;
+    for (uint32_t m = 0; m < n_sources; ++m) {
+        const size_t np = (size_t)sources[m].w * sources[m].h;
+        // RGBA8 rows are the packed u32 layout of the device maps
+        HIP_TRY(hipMemcpyAsync(dpx.p + off, sources[m].rgba, np * 4, hipMemcpyHostToDevice, s));
+        hs[m] = XformSource{dpx.p + off, sources[m].w, sources[m].h};
+        off += np;
+    }
+    if (n_sources) HIP_TRY(hipMemcpyAsync(dsrc.p, hs.data(), n_sources * sizeof(XformSource), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dcoord.p, coord, 3 * n * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemsetAsync(bad.p, 0, 4, s));
+    HIP_TRY(launch_xform_apply(dcoord.p, (uint32_t)n, dsrc.p, n_sources, dout.p, bad.p, s));
+    uint32_t hb = 0;
+    HIP_TRY(hipMemcpyAsync(out, dout.p, n * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&hb, bad.p, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (hb) return fail(TSG_ERR_INVALID_ARGUMENT, "a coordinate lies outside its source map (the reference panics)");
+    return TSG_OK;
+}
+
 int tsg_job_download(tsg_job* job, uint8_t* rgba, uint32_t* coord, uint32_t* ids, uint32_t* resolved_flat,
                      float* resolved_score) {
     if (!job) return fail(TSG_ERR_INVALID_ARGUMENT, "null job");

@@ -133,6 +133,8 @@ def lib():
     L.tsg_job_get_tuning.argtypes = [vp, P(Tuning)]
     L.tsg_ctx_store_tuning.argtypes = [vp, vp, P(Tuning)]
     L.tsg_ctx_clear_tunings.argtypes = [vp]
+    L.tsg_job_debug_maps.argtypes = [vp, vp, vp, vp]
+    L.tsg_xform_apply.argtypes = [vp, vp, u32, u32, u32, P(Image), vp]
     L.tsg_job_run.argtypes = [vp, PROGRESS_FN, vp]
     L.tsg_job_download.argtypes = [vp, vp, vp, vp, vp, vp]
     L.tsg_job_download_packed.argtypes = [vp, vp, vp, vp]
@@ -375,19 +377,20 @@ class CoordinateTransform:
         buf = np.frombuffer(data[off:off + 12 * w * h], "<u4").copy()
         return CoordinateTransform(buf, Dims(w, h), [Dims(int(a), int(b)) for a, b in dims])
 
-    def apply(self, sources: Sequence[np.ndarray]) -> np.ndarray:
+    def apply(self, sources: Sequence[np.ndarray], ctx: Optional["Context"] = None) -> np.ndarray:
         imgs = [as_rgba8(s) for s in sources]
         if len(imgs) != len(self.original_maps):
             raise TsgError(-5, f"{len(imgs)} map(s) were provided, but {len(self.original_maps)} is/are required")
         for im, d in zip(imgs, self.original_maps):
             if im.shape[1] != d.width or im.shape[0] != d.height:
                 raise ValueError("source must match the original map size (resize on the caller's side)")
-        b = self.buffer.reshape(-1, 3)
-        out = np.zeros((self.output_size.height * self.output_size.width, 4), np.uint8)
-        for m, im in enumerate(imgs):
-            sel = b[:, 2] == m
-            out[sel] = im[b[sel, 1], b[sel, 0]]
-        return out.reshape(self.output_size.height, self.output_size.width, 4)
+        # the gather runs on the device (tsg_xform_apply)
+        w, h = self.output_size.width, self.output_size.height
+        out = np.zeros((h, w, 4), np.uint8)
+        srcs = _Images(imgs)
+        coord = np.ascontiguousarray(self.buffer, np.uint32)
+        _check(lib().tsg_xform_apply((ctx or Context.default())._h, _ptr(coord), w, h, len(imgs), srcs.arr, _ptr(out)))
+        return out
 
 
 class Session:
@@ -460,6 +463,18 @@ class Session:
             raise
         self._ran = True
         return self
+
+    def debug_maps(self) -> dict:
+        """Generator::get_id_maps / get_uncertainty_map (ms.rs:603-653) of a job
+        finished by run_resident(), computed on the device: RGBA8 maps
+        "patch_ids", "map_ids" and "uncertainty"."""
+        if self._job is None or not getattr(self, "_ran", False):
+            raise RuntimeError("debug_maps() needs a job finished by run_resident()")
+        w, h = self._out.width, self._out.height
+        maps = {k: np.zeros((h, w, 4), np.uint8) for k in ("patch_ids", "map_ids", "uncertainty")}
+        _check(lib().tsg_job_debug_maps(self._job, _ptr(maps["patch_ids"]), _ptr(maps["map_ids"]),
+                                        _ptr(maps["uncertainty"])))
+        return maps
 
     def download(self) -> GeneratedImage:
         if self._job is None or not getattr(self, "_ran", False):
