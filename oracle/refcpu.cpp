@@ -152,9 +152,16 @@ struct Filter {
     float (*kernel)(float);
     float support;
 };
+/* imageops::blur(image, 2.0): gaussian(x, sigma) with support 2 sigma */
+float k_gaussian_blur2(float x) {
+    const float r = 2.0f;
+    float c = 1.0f / (std::sqrt(2.0f * 3.14159265358979323846f) * r);
+    return c * std::exp(-(x * x) / (2.0f * (r * r)));
+}
 Filter filter_of(int f) {
     if (f == 0) return {k_triangle, 1.0f};
     if (f == 1) return {k_catmullrom, 2.0f};
+    if (f == 3) return {k_gaussian_blur2, 4.0f};
     return {k_gaussian, 3.0f};
 }
 
@@ -936,6 +943,55 @@ Img to_img(const orc_image& im) {
     return r;
 }
 
+/* transform_to_guide_map (utils.rs:101-116) without a resize: blur with
+ * sigma 2 (the resampler at the same size), then grayscale (image 0.23
+ * rgb_to_luma: 0.2126 R + 0.7152 G + 0.0722 B in f32, cast to u8) as RGBA. */
+Img style_guide(const Img& src) {
+    Img b;
+    b.w = src.w;
+    b.h = src.h;
+    b.px.resize(src.px.size());
+    resize_rgba(src.px.data(), src.w, src.h, src.w, src.h, 3, b.px.data());
+    for (size_t i = 0; i < (size_t)b.w * b.h; ++i) {
+        uint8_t* p = &b.px[i * 4];
+        float l = 0.2126f * (float)p[0];
+        l = l + 0.7152f * (float)p[1];
+        l = l + 0.0722f * (float)p[2];
+        const uint8_t v = (uint8_t)l;
+        p[0] = p[1] = p[2] = v;
+        p[3] = 255;
+    }
+    return b;
+}
+
+/* match_histograms / get_histogram / get_cdf (utils.rs:118-183) on the
+ * first channel; the u8 arithmetic wraps as the release build's does. */
+void match_histograms(Img& src, const Img& tgt) {
+    auto cdf_of = [](const Img& im) {
+        std::vector<uint32_t> hist(256, 0);
+        for (size_t i = 0; i < (size_t)im.w * im.h; ++i) hist[im.px[i * 4]] += 1;
+        std::vector<float> c(256, 0.0f);
+        for (int i = 0; i < 256; ++i) c[i] = i ? c[i - 1] + (float)hist[i] : (float)hist[i];
+        const float mx = c[255];
+        for (float& v : c) v /= mx;
+        return c;
+    };
+    const std::vector<float> tc = cdf_of(tgt), sc = cdf_of(src);
+    for (size_t i = 0; i < (size_t)src.w * src.h; ++i) {
+        const uint8_t v = src.px[i * 4];
+        int pos = -1;
+        for (int q = 0; q < 256; ++q)
+            if (tc[q] > sc[v]) {
+                pos = q;
+                break;
+            }
+        const uint8_t n = (uint8_t)((pos >= 0 ? (uint8_t)pos : (uint8_t)(size_t)(uint8_t)(v + 1)) - 1u);
+        uint8_t* p = &src.px[i * 4];
+        p[0] = p[1] = p[2] = n;
+        p[3] = 255;
+    }
+}
+
 }  // namespace
 
 /* ------------------------------------------------------------------ */
@@ -977,6 +1033,27 @@ void orc_luts_full(float cauchy_dispersion, float alpha_s, int guided, float* my
             my[p * 256 + q] = guided ? (1.0f - alpha_s) * c : c;
             gc[p * 256 + q] = guided ? alpha_s * l2 : 0.0f;
         }
+}
+
+void orc_guide_map(const uint8_t* src, uint32_t w, uint32_t h, uint8_t* dst) {
+    Img in;
+    in.w = w;
+    in.h = h;
+    in.px.assign(src, src + (size_t)w * h * 4);
+    const Img g = style_guide(in);
+    memcpy(dst, g.px.data(), g.px.size());
+}
+
+void orc_match_histograms(uint8_t* src, uint32_t w, uint32_t h, const uint8_t* tgt, uint32_t tw, uint32_t th) {
+    Img s, t;
+    s.w = w;
+    s.h = h;
+    s.px.assign(src, src + (size_t)w * h * 4);
+    t.w = tw;
+    t.h = th;
+    t.px.assign(tgt, tgt + (size_t)tw * th * 4);
+    match_histograms(s, t);
+    memcpy(src, s.px.data(), s.px.size());
 }
 
 int orc_resize(const uint8_t* src, uint32_t w, uint32_t h, uint32_t nw, uint32_t nh, int filter, uint8_t* dst) {
@@ -1027,19 +1104,26 @@ orc_job* orc_job_new(const orc_params* prm, uint32_t n_examples, const orc_image
     }
     if (target_guide) {
         j->guided = true;
-        if (!example_guides) {
-            j->err = "example guides are required with a target guide";
-            return j;
-        }
+        /* style transfer (session.rs:385-393, lib.rs:572-581): without example
+         * guides the target guide becomes a guide map, and each example's
+         * guide is its own guide map matched to the target's histogram */
+        const Img tg = example_guides ? to_img(*target_guide) : style_guide(to_img(*target_guide));
         for (uint32_t i = 0; i < n_examples; ++i) {
+            Img g;
+            if (example_guides) {
+                g = to_img(example_guides[i]);
+            } else {
+                g = style_guide(to_img(examples[i]));
+                match_histograms(g, tg);
+            }
             std::vector<Img> pyr;
-            if (!build_pyramid(to_img(example_guides[i]), j->levels, pyr)) {
+            if (!build_pyramid(g, j->levels, pyr)) {
                 j->err = "guide too small";
                 return j;
             }
             j->exg.push_back(std::move(pyr));
         }
-        if (!build_pyramid(to_img(*target_guide), j->levels, j->tg)) {
+        if (!build_pyramid(tg, j->levels, j->tg)) {
             j->err = "target guide too small";
             return j;
         }

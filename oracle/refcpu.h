@@ -38,6 +38,10 @@ void orc_luts_full(float cauchy_dispersion, float alpha_s, int guided, float* my
 /* ---- image resampling (image 0.23.12 imageops::resize, restated) ---- */
 /* filter: 0 = Triangle, 1 = CatmullRom, 2 = Gaussian. RGBA8 in/out. */
 int orc_resize(const uint8_t* src, uint32_t w, uint32_t h, uint32_t nw, uint32_t nh, int filter, uint8_t* dst);
+/* Style-transfer guide steps (utils.rs:101-183): transform_to_guide_map
+ * without a resize (blur sigma 2, grayscale) and match_histograms, in place. */
+void orc_guide_map(const uint8_t* src, uint32_t w, uint32_t h, uint8_t* dst);
+void orc_match_histograms(uint8_t* src, uint32_t w, uint32_t h, const uint8_t* tgt, uint32_t tw, uint32_t th);
 /* ImagePyramid::new(img, Some(levels)): writes `levels` (>=1) full-size RGBA8 levels. */
 int orc_pyramid(const uint8_t* src, uint32_t w, uint32_t h, uint32_t levels, uint8_t* out);
 

@@ -89,6 +89,10 @@ def lib():
         L.orc_luts_full.argtypes = [C.c_float, C.c_float, C.c_int, C.c_void_p, C.c_void_p]
         L.orc_resize.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_void_p]
         L.orc_pyramid.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]
+        L.orc_guide_map.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
+        L.orc_guide_map.restype = None
+        L.orc_match_histograms.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32]
+        L.orc_match_histograms.restype = None
         L.orc_knn.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32,
                               C.c_void_p, C.c_void_p]
         _lib = L
@@ -151,6 +155,21 @@ def resize(img: np.ndarray, nw: int, nh: int, filt: str) -> np.ndarray:
     out = np.zeros((nh, nw, 4), np.uint8)
     rc = lib().orc_resize(_ptr(img), w, h, nw, nh, FILTERS[filt], _ptr(out))
     assert rc == 0
+    return out
+
+
+def guide_map(img: np.ndarray) -> np.ndarray:
+    """transform_to_guide_map without a resize: blur sigma 2, grayscale, RGBA."""
+    img = np.ascontiguousarray(img, np.uint8)
+    out = np.zeros_like(img)
+    lib().orc_guide_map(_ptr(img), img.shape[1], img.shape[0], _ptr(out))
+    return out
+
+
+def match_histograms(src: np.ndarray, tgt: np.ndarray) -> np.ndarray:
+    out = np.ascontiguousarray(src, np.uint8).copy()
+    tgt = np.ascontiguousarray(tgt, np.uint8)
+    lib().orc_match_histograms(_ptr(out), out.shape[1], out.shape[0], _ptr(tgt), tgt.shape[1], tgt.shape[0])
     return out
 
 

@@ -167,6 +167,12 @@ float kern_catmullrom(float x) {
         k = 0.0f;
     return k / 6.0f;
 }
+// imageops::blur(image, 2.0): gaussian(x, sigma) with support 2 sigma
+float kern_gaussian_blur2(float x) {
+    const float r = 2.0f;
+    float c = 1.0f / (std::sqrt(2.0f * 3.14159265358979323846f) * r);
+    return c * std::exp(-(x * x) / (2.0f * (r * r)));
+}
 float kern_gaussian(float x) {
     // gaussian(x, r = 0.5)
     const float r = 0.5f;
@@ -244,6 +250,64 @@ int device_resize(const uint32_t* src, uint32_t w, uint32_t h, uint32_t* dst, ui
     HIP_TRY(launch_resample_h(tmp.p, w, nh, dst, nw, dh.left.p, dh.cnt.p, dh.ws.p, dh.stride, s));
     HIP_TRY(hipStreamSynchronize(s));  // taps and tmp die here
     return TSG_OK;
+}
+
+// transform_to_guide_map (utils.rs:101-116) without a resize: blur with
+// sigma 2 on the device (the resampler at the same size), then grayscale on
+// the host (image 0.23 rgb_to_luma: 0.2126 R + 0.7152 G + 0.0722 B in f32,
+// cast to u8), as RGBA.
+int style_guide(hipStream_t s, const tsg_image& im, std::vector<uint8_t>& out) {
+    const size_t n = (size_t)im.w * im.h;
+    DBuf<uint32_t> src, dst;
+    HIP_TRY(src.alloc(n));
+    HIP_TRY(dst.alloc(n));
+    HIP_TRY(hipMemcpyAsync(src.p, im.rgba, n * 4, hipMemcpyHostToDevice, s));
+    const Filter blur{kern_gaussian_blur2, 4.0f};
+    int rc = device_resize(src.p, im.w, im.h, dst.p, im.w, im.h, blur, s);
+    if (rc) return rc;
+    out.resize(n * 4);
+    HIP_TRY(hipMemcpy(out.data(), dst.p, n * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) {
+        uint8_t* p = &out[i * 4];
+        float l = 0.2126f * (float)p[0];
+        l = l + 0.7152f * (float)p[1];
+        l = l + 0.0722f * (float)p[2];
+        const uint8_t v = (uint8_t)l;
+        p[0] = p[1] = p[2] = v;
+        p[3] = 255;
+    }
+    return TSG_OK;
+}
+
+// match_histograms / get_histogram / get_cdf (utils.rs:118-183) on the first
+// channel: each value moves to the position before the first target CDF entry
+// above its own CDF; the u8 arithmetic wraps as the release build's does.
+void match_histograms(std::vector<uint8_t>& src, const std::vector<uint8_t>& tgt) {
+    auto cdf_of = [](const std::vector<uint8_t>& im) {
+        std::vector<uint32_t> hist(256, 0);
+        for (size_t i = 0; i < im.size(); i += 4) hist[im[i]] += 1;
+        std::vector<float> c(256, 0.0f);
+        for (int i = 0; i < 256; ++i) c[i] = i ? c[i - 1] + (float)hist[i] : (float)hist[i];
+        const float mx = c[255];
+        for (float& v : c) v /= mx;
+        return c;
+    };
+    const std::vector<float> tc = cdf_of(tgt), sc = cdf_of(src);
+    uint8_t map[256];
+    for (int v = 0; v < 256; ++v) {
+        int pos = -1;
+        for (int q = 0; q < 256; ++q)
+            if (tc[q] > sc[v]) {
+                pos = q;
+                break;
+            }
+        map[v] = (uint8_t)((pos >= 0 ? (uint8_t)pos : (uint8_t)(v + 1)) - 1u);
+    }
+    for (size_t i = 0; i < src.size(); i += 4) {
+        const uint8_t n = map[src[i]];
+        src[i] = src[i + 1] = src[i + 2] = n;
+        src[i + 3] = 255;
+    }
 }
 
 // ImagePyramid::build_gaussian (img_pyramid.rs:20-37): `levels` full-size
@@ -1897,8 +1961,6 @@ int tsg_job_create(tsg_ctx* ctx, const tsg_params* prm, uint32_t n_examples, con
     if (prm->nearest_neighbors == 0 || prm->nearest_neighbors > (uint32_t)kMaxK)
         return fail(TSG_ERR_UNSUPPORTED, "nearest_neighbors must be in 1..64 on the device");
     if (prm->random_sample_locations > 4096) return fail(TSG_ERR_UNSUPPORTED, "random_sample_locations above 4096");
-    if (target_guide && !example_guides)
-        return fail(TSG_ERR_UNSUPPORTED, "style transfer (target guide without example guides) needs host guide prep");
     if (out_w == 0 || out_h == 0 || out_w > 16384 || out_h > 16384)
         return fail(TSG_ERR_INVALID_ARGUMENT, "output size must be 1..16384 per side");
     for (uint32_t i = 0; i < n_examples; ++i) {
@@ -1950,6 +2012,26 @@ int tsg_job_create(tsg_ctx* ctx, const tsg_params* prm, uint32_t n_examples, con
         return fail(TSG_ERR_UNSUPPORTED, "inpaint example index beyond the sampled examples (the reference panics)");
     hipStream_t s = ctx->stream;
     std::lock_guard<std::mutex> lk(ctx->mu);
+    // style transfer (session.rs:385-393, lib.rs:572-581): without example
+    // guides the target guide becomes a guide map, and each example's guide
+    // is its own guide map matched to the target's histogram
+    std::vector<std::vector<uint8_t>> sg_ex;
+    std::vector<uint8_t> sg_tg;
+    std::vector<tsg_image> sg_imgs;
+    tsg_image sg_target{};
+    if (target_guide && !example_guides) {
+        if ((rc = style_guide(s, *target_guide, sg_tg))) return rc;
+        sg_target = tsg_image{target_guide->w, target_guide->h, sg_tg.data()};
+        sg_ex.resize(n_examples);
+        sg_imgs.resize(n_examples);
+        for (uint32_t i = 0; i < n_examples; ++i) {
+            if ((rc = style_guide(s, examples[i], sg_ex[i]))) return rc;
+            match_histograms(sg_ex[i], sg_tg);
+            sg_imgs[i] = tsg_image{examples[i].w, examples[i].h, sg_ex[i].data()};
+        }
+        example_guides = sg_imgs.data();
+        target_guide = &sg_target;
+    }
     for (uint32_t i = 0; i < n_examples; ++i) {
         const tsg_image& im = examples[i];
         j->ex_dims.push_back({im.w, im.h});
