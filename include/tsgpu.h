@@ -197,9 +197,10 @@ int tsg_xform_apply(tsg_ctx* ctx, const uint32_t* coord, uint32_t w, uint32_t h,
                     const tsg_image* sources, uint8_t* out);
 int tsg_job_download(tsg_job* job, uint8_t* rgba, uint32_t* coord, uint32_t* ids, uint32_t* resolved_flat,
                      float* resolved_score);
-/* Per stage (index 0 = p_stage P): sum of the winning costs of
- * pattern-matched items, their count, and items processed. Returns the
- * number of stages written (<= cap). */
+/* Per stage (index 0 = p_stage P): sum of the winning costs of items
+ * resolved by pattern matching with a numeric cost (NaN weights, from a zero
+ * mean neighbour distance, give NaN costs), their count, and items
+ * processed. Returns the number of stages written (<= cap). */
 int tsg_job_stage_stats(tsg_job* job, double* sums, uint64_t* counts, uint64_t* items, uint32_t cap);
 /* Device time of the last tsg_job_run in milliseconds, and batch count. */
 int tsg_job_timing(tsg_job* job, double* run_ms, uint64_t* n_batches);

@@ -905,7 +905,10 @@ bool run_batch(orc_job* j, uint64_t e) {
         const Result& r = res[q];
         write_pixel(j, r.flat, r.sx, r.sy, r.smap, r.patch, r.pmap, j->maps);
         if (r.is_new) flush(j, r.flat, r.score, !r.fallback && j->prm.tiling_mode);
-        if (!r.fallback) {
+        /* per-stage statistics (not a reference quantity): items resolved by
+         * pattern matching with a numeric cost (NaN weights, from a zero mean
+         * neighbour distance, give NaN costs) */
+        if (!r.fallback && r.score >= 0.0f) {
             j->ssum[j->t] += r.score;
             j->scnt[j->t] += 1;
         }

@@ -102,7 +102,8 @@ uint64_t orc_job_locked(orc_job* j);
 /* Generator::get_id_maps / get_uncertainty_map (ms.rs:603-653) of the
  * current state: RGBA8 w*h*4 each, any may be NULL. */
 void orc_job_debug_maps(orc_job* j, uint8_t* patch_ids, uint8_t* map_ids, uint8_t* uncertainty);
-/* per-stage: sum of best scores, count of pattern-matched items, items done. */
+/* per-stage: sum and count of the best scores of items resolved by pattern
+ * matching with a numeric (non-NaN) cost, items done. */
 uint32_t orc_job_stage_stats(orc_job* j, double* sums, uint64_t* counts, uint64_t* items);
 /* resolved mask (1 byte per px) */
 void orc_job_mask(orc_job* j, uint8_t* mask);
