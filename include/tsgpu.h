@@ -99,7 +99,10 @@ typedef struct tsg_schedule {
  *                new-pixel batches goes to one launch of persistent
  *                workgroups that resolve the items in order, each waiting
  *                only for the neighbours committed inside the run (per-pixel
- *                flags); 0: one launch (or loop step) per batch
+ *                flags); 2: as 1, and the leading run of latency-bound redo
+ *                batches too, reading neighbours not redone before them from
+ *                a snapshot of the state taken as the stage opens; 0: one
+ *                launch (or loop step) per batch
  *   flow_wgs     workgroups of that launch (0: as many as are resident)
  *   flow_team    waves per item in that launch (1, 2, 4 or 8) */
 typedef struct tsg_tuning {

@@ -19,7 +19,7 @@ DEFAULT = ts.default_schedule()
 
 
 def run_pair(examples, out_w, out_h, sched=DEFAULT, threads=8, sample_kind=None, sample_masks=None,
-             example_guides=None, target_guide=None, inpaint_mask=None, inpaint_index=0, **params):
+             example_guides=None, target_guide=None, inpaint_mask=None, inpaint_index=0, tuning=None, **params):
     """Same job on the device (through the Session mirror) and on the oracle."""
     prm = R.default_params(**params)
     job = R.Job(prm, examples, out_w, out_h, sample_kind=sample_kind, sample_masks=sample_masks,
@@ -55,6 +55,8 @@ def run_pair(examples, out_w, out_h, sched=DEFAULT, threads=8, sample_kind=None,
         getattr(b, k)(v)
     sess = b.build()
     sess.set_schedule(**sched)
+    if tuning:
+        sess.set_tuning(**tuning)
     gen = sess.run()
     return gen, job
 
@@ -244,6 +246,10 @@ def test_default_schedule_bit_exact_c1(ctx, seed):
     dict(flow=1, flow_team=1, flow_wgs=3),
     dict(flow=1, flow_team=2, flow_wgs=1),
     dict(flow=1, flow_team=4, prefetch_max=64),
+    dict(flow=2),
+    dict(flow=2, flow_team=1, flow_wgs=3),
+    dict(flow=2, flow_team=2, flow_wgs=1, prefetch_max=64),
+    dict(flow=2, ahead=2),
 ])
 def test_launch_tuning_does_not_change_results(ctx, tuning):
     # every launch structure resolves the same batches against the same states
@@ -254,6 +260,26 @@ def test_launch_tuning_does_not_change_results(ctx, tuning):
     sess = ts.Session.builder().add_example(ex).seed(211).output_size(ts.Dims.square(256)).build()
     sess.set_schedule(**DEFAULT).set_tuning(**tuning)
     assert_same(sess.run(), job)
+
+
+@pytest.mark.parametrize("flow", [1, 2])
+def test_flow_runs_guided_random_init_bit_exact(ctx, flow):
+    # redo runs read locked (random-init) pixels, earlier-batch redos and
+    # not-yet-redone pixels from three different sources
+    exs = [synth.example(1300 + i, 56, 56) for i in range(2)]
+    gds = [synth.blobs(1310 + i, 56, 56) for i in range(2)]
+    tg = synth.blobs(1320, 200, 168)
+    gen, job = run_pair(exs, 200, 168, seed=5, example_guides=gds, target_guide=tg, random_init=40,
+                        guide_alpha=0.7, tuning=dict(flow=flow))
+    assert_same(gen, job)
+
+
+def test_redo_runs_at_512_bit_exact(ctx):
+    # at 512x512 the third stage's redo batches (about 600 items) still fit
+    # one dependency-ordered run
+    ex = synth.example(1330, 128, 128)
+    gen, job = run_pair([ex], 512, 512, seed=13, tuning=dict(flow=2, prefetch_max=1024))
+    assert_same(gen, job)
 
 
 def test_multi_example_random_init_bit_exact(ctx):

@@ -100,3 +100,11 @@ def test_degenerate_outputs_bit_exact(ctx, out_w, out_h):
 def test_tiny_ragged_example_bit_exact(ctx):
     gen, job = run_pair([synth.example(30, 17, 16)], 40, 23, seed=4, backtrack_stages=3)
     assert_same(gen, job)
+
+
+@pytest.mark.parametrize("knob,value", [("flow", 3), ("flow_team", 3), ("ahead", 3), ("team_max", 16)])
+def test_tuning_out_of_range(ctx, knob, value):
+    s = builder().build()
+    with pytest.raises(ts.TsgError) as e:
+        s.set_tuning(**{knob: value})
+    assert e.value.kind == "InvalidArgument"

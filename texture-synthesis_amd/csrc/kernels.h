@@ -84,6 +84,10 @@ struct PrefetchArgs {
     const uint32_t* bounds;       // batch b holds items [bounds[b], bounds[b + 1]) (nb + 1 entries)
     const uint32_t* bnres;        // and starts with bnres[b] resolved pixels
     uint32_t nb;
+    // items below redo_end redo resolved pixels: their dependency bits mark
+    // neighbours redone by an earlier batch (rank - locked below the item's
+    // batch start); the other items' bits mark pixels of mask_new
+    uint32_t redo_end, locked;
     const int32_t* offtab;
     uint32_t offtab_n;
     uint32_t k;
@@ -94,8 +98,9 @@ struct PrefetchArgs {
     uint64_t* out_dep;            // [item - item0]
 };
 
-// A run of consecutive new-pixel items [a.item0, a.item0 + n) of one stage in
-// one launch of persistent workgroups. Every item has its neighbour list
+// A run of consecutive items [a.item0, a.item0 + n) of one segment (new
+// pixels, or redo items with state_old) in one launch of persistent
+// workgroups. Every item has its neighbour list
 // (a.pre_nbrs) and a dependency mask dep[item - a.pre_base]: bit j set when
 // neighbour j is a pixel committed by an earlier item of the run. Workgroups
 // take items in order from a ticket counter, wait until each dependency's
@@ -113,6 +118,10 @@ struct FlowArgs {
     uint32_t epoch;
     uint32_t spin_limit;
     uint32_t* overflow;
+    // a run of redo items: the state as the segment opened; neighbours
+    // without a dependency bit are read there (the run rewrites the live
+    // state), and commits touch neither mask nor score. Null: new pixels.
+    const uint4* state_old;
 };
 
 struct RandArgs {
@@ -231,6 +240,7 @@ hipError_t launch_resolve_flow(const BatchArgs& a, const FlowArgs& f, uint32_t w
 hipError_t launch_stats(const StatsArgs& a, hipStream_t s);
 hipError_t launch_rand(const RandArgs& a, hipStream_t s);
 hipError_t launch_commit(const CommitArgs& a, hipStream_t s);
+hipError_t launch_state_snapshot(const uint32_t* rlist, uint32_t n, const uint4* state, uint4* old, hipStream_t s);
 hipError_t launch_recolor(const RecolorArgs& a, hipStream_t s);
 // separable resampling pass: vertical (u8 -> f32) and horizontal (f32 -> u8)
 hipError_t launch_resample_v(const uint32_t* src, uint32_t w, uint32_t h, float4* dst, uint32_t nh,

@@ -846,11 +846,13 @@ __device__ __forceinline__ void item_geometry(const BatchArgs& a, int x, int y, 
     wave_sync();
 }
 
+// old_state (redo runs): neighbours without a bit in `fresh` are read there.
 template <bool GUIDED>
 __device__ __forceinline__ uint32_t item_neighbours(const BatchArgs& a, const MapDesc* maps, uint32_t item,
                                                     uint32_t rel, int x, int y, uint32_t kf, uint32_t n_k0,
                                                     WaveLds* L, int lane, bool geo_done,
-                                                    uint32_t& xy, uint32_t& ms, uint64_t& uniq) {
+                                                    uint32_t& xy, uint32_t& ms, uint64_t& uniq,
+                                                    const uint4* old_state = nullptr, uint64_t fresh = 0) {
     bool coh_valid = false;
     uint4 coh = make_uint4(0, 0, 0, 0);
     if (lane < (int)kf) {
@@ -862,7 +864,8 @@ __device__ __forceinline__ uint32_t item_neighbours(const BatchArgs& a, const Ma
             ny = ny < 0 ? ny + (int)a.H : (ny >= (int)a.H ? ny - (int)a.H : ny);
         }
         uint32_t nflat = (uint32_t)ny * a.W + (uint32_t)nx;
-        uint4 st = a.state[nflat];
+        const uint4* src = (old_state && !((fresh >> lane) & 1ull)) ? old_state : a.state;
+        uint4 st = src[nflat];
         L->pat[lane] = st.x;
         if (GUIDED) L->gpat[lane] = a.tg[nflat];
         if (!a.ext_cands) {
@@ -1131,7 +1134,10 @@ __device__ uint4 resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t 
         // ---- 2. neighbour state, weights, coherence candidates (every member) ----
         uint32_t cxy, cms;
         uint64_t uniq;
-        n_coh = item_neighbours<GUIDED>(a, maps, item, rel, x, y, kf, n_k0, L, lane, pre, cxy, cms, uniq);
+        const uint4* old_state = flow ? flow->state_old : nullptr;
+        const uint64_t fresh = old_state ? flow->dep[item - a.pre_base] : 0ull;
+        n_coh = item_neighbours<GUIDED>(a, maps, item, rel, x, y, kf, n_k0, L, lane, pre, cxy, cms, uniq, old_state,
+                                        fresh);
         if (!a.ext_cands) {
             n_k0 = n_coh;
             n_c = n_coh + a.m;
@@ -1429,10 +1435,13 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve_flow(BatchArgs a
             const uint32_t color = md.px[(r.x >> 16) * md.w + (r.x & 0xffffu)];
             a.state_w[p] = make_uint4(color, r.x, r.y & ~kFallbackBit, r.z);
             __hip_atomic_store(f.flag + p, f.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            const uint32_t px = p % a.W, py = p / a.W;
-            atomicOr((unsigned long long*)&a.mask_w[(py >> 3) * a.tiles_x + (px >> 3)],
-                     1ull << (((py & 7u) << 3) | (px & 7u)));
-            a.score_w[p] = __uint_as_float(r.w);
+            if (!f.state_old) {
+                // a new pixel enters the resolved mask and keeps its first score
+                const uint32_t px = p % a.W, py = p / a.W;
+                atomicOr((unsigned long long*)&a.mask_w[(py >> 3) * a.tiles_x + (px >> 3)],
+                         1ull << (((py & 7u) << 3) | (px & 7u)));
+                a.score_w[p] = __uint_as_float(r.w);
+            }
             phase_clock(a, item, 10, 0);
         }
     }
@@ -1493,7 +1502,7 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_knn_prefetch(PrefetchArg
     if (item >= a.item1) return;
     const uint32_t rel = item - a.item0;
     const uint32_t flat = a.work[item];
-    uint32_t n_at;
+    uint32_t n_at, bs = 0;
     if (a.n_at) {
         n_at = a.n_at[rel];
     } else {
@@ -1504,6 +1513,7 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_knn_prefetch(PrefetchArg
             else hi = mid - 1;
         }
         n_at = a.bnres[lo];
+        bs = a.bounds[lo];
     }
     const KnnView v{a.W,        a.H,      a.tiles_x, a.tiles_y, a.mask, a.rlist, a.offtab, a.offtab_n,
                     a.mask_new, a.tile_min, a.rank, n_at};
@@ -1513,9 +1523,16 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_knn_prefetch(PrefetchArg
     if (lane == 0) a.out_cnt[rel] = kf;
     if (a.out_dep) {
         bool d = false;
-        if (lane < (int)kf && a.mask_new) {
+        if (lane < (int)kf) {
             const int32_t o = L->nd[lane];
-            d = px_set(a.mask_new, a.tiles_x, (int)(flat % a.W) + packed_dx(o), (int)(flat / a.W) + packed_dy(o));
+            const int X = (int)(flat % a.W) + packed_dx(o), Y = (int)(flat / a.W) + packed_dy(o);
+            if (item < a.redo_end) {
+                // redone by an earlier batch of the segment
+                const uint32_t r = a.rank[(uint32_t)Y * a.W + (uint32_t)X];
+                d = r >= a.locked && r - a.locked < bs;
+            } else if (a.mask_new) {
+                d = px_set(a.mask_new, a.tiles_x, X, Y);
+            }
         }
         const uint64_t dm = __ballot(d);
         if (lane == 0) a.out_dep[rel] = dm;
@@ -1642,6 +1659,14 @@ __global__ void k_commit(CommitArgs a) {
 }
 
 // next_pyramid_level (ms.rs:687-700): recolour every resolved pixel.
+// old[p] = state[p] for the first n pixels of the resolved list.
+__global__ void k_state_snapshot(const uint32_t* rlist, uint32_t n, const uint4* state, uint4* old) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t p = rlist[i];
+    old[p] = state[p];
+}
+
 __global__ void k_recolor(RecolorArgs a) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
@@ -1977,6 +2002,12 @@ hipError_t launch_commit(const CommitArgs& a, hipStream_t s) {
     uint32_t n = a.item1 - a.item0;
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_commit, dim3((n + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_state_snapshot(const uint32_t* rlist, uint32_t n, const uint4* state, uint4* old, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_state_snapshot, dim3((n + 255) / 256), dim3(256), 0, s, rlist, n, state, old);
     return hipGetLastError();
 }
 

@@ -626,6 +626,7 @@ struct tsg_job {
     // it, the runs' ticket counter and its value as the next run starts
     DBuf<uint32_t> flow_flag;
     DBuf<uint32_t> flow_ticket;
+    DBuf<uint4> state_old;  // redo runs: the state as the redo segment opened
     uint32_t flow_epoch = 0, flow_tickets = 0;
     size_t small_used = 0;
     // host bookkeeping
@@ -1053,6 +1054,8 @@ int ahead_stage(tsg_job* j) {
         pa.bounds = A.n_at.p;
         pa.bnres = A.n_at.p + nb + 1;
         pa.nb = nb;
+        pa.redo_end = (uint32_t)redo_eff;
+        pa.locked = (uint32_t)j->locked;
         pa.offtab = j->ctx->offtab.p;
         pa.offtab_n = j->ctx->offtab_n;
         pa.k = k;
@@ -1478,6 +1481,7 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
             HIP_TRY(hipMemsetAsync(j->flow_ticket.p, 0, 4, s));
             j->flow_tickets = 0;
         }
+        if (j->tune.flow >= 2 && j->state_old.n < j->N) HIP_TRY(j->state_old.alloc(j->N));
     }
     int last_pct = 0;
     std::vector<uint8_t> snap;
@@ -1492,6 +1496,37 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
         HIP_TRY(hipMemcpy(stv.data(), j->state.p, j->N * sizeof(uint4), hipMemcpyDeviceToHost));
         for (size_t q = 0; q < j->N; ++q) memcpy(&snap[q * 4], &stv[q].x, 4);
         cb(user, snap.data(), j->W, j->H, j->overall_cur + i1, j->overall_total, i1, j->step.S);
+        return TSG_OK;
+    };
+    // items [i0, end) of the open stage in one dependency-ordered launch;
+    // old: a redo run, which reads neighbours not redone before it from the
+    // state as the segment opened
+    auto flow_run = [&](uint64_t i0, uint64_t end, const uint4* old) -> int {
+        tsg_job::Step& st = j->step;
+        BatchArgs& a = st.a;
+        a.item0 = (uint32_t)i0;
+        a.item1 = (uint32_t)end;
+        a.n_res = (uint32_t)j->n_res;
+        if (++j->flow_epoch == 0) {
+            HIP_TRY(hipMemsetAsync(j->flow_flag.p, 0, j->N * 4, s));
+            j->flow_epoch = 1;
+        }
+        const uint32_t n = (uint32_t)(end - i0);
+        const uint32_t g = std::min(flow_wgs, n);
+        const FlowArgs f{j->flow_ticket.p, j->flow_tickets, n, j->aset(st.t).dep.p, j->flow_flag.p,
+                         j->flow_epoch, kFlowSpin, j->overflow.p, old};
+        HIP_TRY(timed_launch(j, K_RUN, n, [&] {
+            BatchArgs fa = a;  // results stay in registers
+            fa.result = nullptr;
+            return launch_resolve_flow(fa, f, g, flow_team, j->guided, s);
+        }));
+        j->flow_tickets += n + g;
+        while (st.i < end) {
+            const uint64_t b0 = st.i, b1 = batch_end(j, b0, st.redo, st.S_eff);
+            step_advance(j, b0, b1);
+        }
+        if (st.i != end) return fail(TSG_ERR_DEVICE, "dependency-ordered run out of step");
+        if (cb) return progress(end);
         return TSG_OK;
     };
     std::vector<uint32_t> sb_bounds, sb_nres;
@@ -1532,34 +1567,19 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
         while (step_peek(j, i0, i1)) {
             tsg_job::Step& st = j->step;
             const uint64_t flow_end = flow_on ? std::min<uint64_t>(j->aset(st.t).w_end, st.S_eff) : 0;
+            if (flow_on && j->tune.flow >= 2 && i0 == 0 && st.redo_eff > 0 && flow_end > 0) {
+                // the stage's leading redo batches: one run over a snapshot
+                // of the resolved pixels' state
+                if ((rc = flush_small())) return rc;
+                HIP_TRY(launch_state_snapshot(j->rlist.p, (uint32_t)j->n_res, j->state.p, j->state_old.p, s));
+                if ((rc = flow_run(0, std::min(flow_end, st.redo_eff), j->state_old.p))) return rc;
+                continue;
+            }
             if (flow_on && i0 == st.redo_eff && flow_end > i0) {
                 // the stage's new-pixel batches with neighbour lists: one
                 // dependency-ordered run
                 if ((rc = flush_small())) return rc;
-                BatchArgs& a = st.a;
-                a.item0 = (uint32_t)i0;
-                a.item1 = (uint32_t)flow_end;
-                a.n_res = (uint32_t)j->n_res;
-                if (++j->flow_epoch == 0) {
-                    HIP_TRY(hipMemsetAsync(j->flow_flag.p, 0, j->N * 4, s));
-                    j->flow_epoch = 1;
-                }
-                const uint32_t n = (uint32_t)(flow_end - i0);
-                const uint32_t g = std::min(flow_wgs, n);
-                const FlowArgs f{j->flow_ticket.p, j->flow_tickets, n, j->aset(st.t).dep.p,
-                                 j->flow_flag.p, j->flow_epoch, kFlowSpin, j->overflow.p};
-                HIP_TRY(timed_launch(j, K_RUN, n, [&] {
-                    BatchArgs fa = a;  // results stay in registers
-                    fa.result = nullptr;
-                    return launch_resolve_flow(fa, f, g, flow_team, j->guided, s);
-                }));
-                j->flow_tickets += n + g;
-                while (st.i < flow_end) {
-                    const uint64_t b0 = st.i, b1 = batch_end(j, b0, st.redo, st.S_eff);
-                    step_advance(j, b0, b1);
-                }
-                if (st.i != flow_end) return fail(TSG_ERR_DEVICE, "dependency-ordered run out of step");
-                if (cb && (rc = progress(flow_end))) return rc;
+                if ((rc = flow_run(i0, flow_end, nullptr))) return rc;
                 continue;
             }
             if (step_wants_prefetch(j, i0, i1)) {
@@ -1616,6 +1636,7 @@ int check_tuning(const tsg_tuning* t) {
         return fail(TSG_ERR_INVALID_ARGUMENT, "team_max must be 1, 2, 4 or 8");
     if (t->coop_max > t->coop_wgs * small_batch_max())
         return fail(TSG_ERR_INVALID_ARGUMENT, "coop_max above 16 items per cooperative workgroup");
+    if (t->flow > 2) return fail(TSG_ERR_INVALID_ARGUMENT, "flow must be 0, 1 or 2");
     if (t->flow && (t->flow_team == 0 || t->flow_team > 8 || (t->flow_team & (t->flow_team - 1))))
         return fail(TSG_ERR_INVALID_ARGUMENT, "flow_team must be 1, 2, 4 or 8");
     if (t->ahead > 2) return fail(TSG_ERR_INVALID_ARGUMENT, "ahead must be 0, 1 or 2");
@@ -1636,7 +1657,7 @@ size_t job_footprint(const tsg_job* j) {
     for (const tsg_job::Ahead& A : j->ahead)
         b += A.rc.n * sizeof(uint2) + A.nbrs.n * 4 + A.cnt.n * 4 + A.n_at.n * 4 + A.mend.n * 8 + A.mnew.n * 8 +
              A.tmin.n * 4 + A.dep.n * 8;
-    b += j->flow_flag.n * 4 + j->flow_ticket.n * 4;
+    b += j->flow_flag.n * 4 + j->flow_ticket.n * 4 + j->state_old.n * sizeof(uint4);
     return b;
 }
 

@@ -21,7 +21,7 @@ from . import Context, SessionBuilder, default_tuning
 
 # knobs in search order, with the values tried for each
 SPACE: Sequence[Tuple[str, Sequence[int]]] = (
-    ("flow", (1, 0)),
+    ("flow", (1, 2, 0)),
     ("flow_team", (2, 4, 8)),
     ("team_max", (4, 8)),
     ("small_max", (0, 4, 16)),
