@@ -28,7 +28,10 @@ constexpr int kWave = 64;
 constexpr int kWavesPerWG = 8;
 constexpr int kSmallWaves = 16;  // items per batch of k_resolve_small
 constexpr int kKeyCap = 256;
-constexpr int kLutCopies = 32;
+// Interleaved copies of each 256-entry cost table in LDS (lane l reads copy
+// l % kLutCopies). 16 copies keep two 8-wave workgroups resident per CU with
+// both tables (guided jobs); 32 fitted one, at 26% more guided device time.
+constexpr int kLutCopies = 16;
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
