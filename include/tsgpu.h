@@ -104,7 +104,12 @@ typedef struct tsg_schedule {
  *                a snapshot of the state taken as the stage opens; 0: one
  *                launch (or loop step) per batch
  *   flow_wgs     workgroups of that launch (0: as many as are resident)
- *   flow_team    waves per item in that launch (1, 2, 4 or 8) */
+ *   flow_team    waves per item in that launch (1, 2, 4 or 8)
+ *   defer        1: in a single-job run, a throughput-bound batch's commit is
+ *                done by the next such batch's launch (its items read the
+ *                previous batch's records from a second state array), so a
+ *                series of batches needs one launch each instead of two;
+ *                0 (default): a commit launch after every batch */
 typedef struct tsg_tuning {
     uint32_t small_max;
     uint32_t fuse_max;
@@ -117,6 +122,7 @@ typedef struct tsg_tuning {
     uint32_t flow;
     uint32_t flow_wgs;
     uint32_t flow_team;
+    uint32_t defer;
 } tsg_tuning;
 
 typedef struct tsg_image {

@@ -250,6 +250,10 @@ def test_default_schedule_bit_exact_c1(ctx, seed):
     dict(flow=2, flow_team=1, flow_wgs=3),
     dict(flow=2, flow_team=2, flow_wgs=1, prefetch_max=64),
     dict(flow=2, ahead=2),
+    dict(defer=0),
+    dict(defer=1, flow=0, ahead=0, prefetch_max=0, small_max=0, team_max=1),
+    dict(defer=1, flow=0, ahead=0, prefetch_max=64, small_max=16, fuse_max=64),
+    dict(defer=1, coop_wgs=8, coop_max=128, ahead=0),
 ])
 def test_launch_tuning_does_not_change_results(ctx, tuning):
     # every launch structure resolves the same batches against the same states
@@ -280,6 +284,24 @@ def test_redo_runs_at_512_bit_exact(ctx):
     ex = synth.example(1330, 128, 128)
     gen, job = run_pair([ex], 512, 512, seed=13, tuning=dict(flow=2, prefetch_max=1024))
     assert_same(gen, job)
+
+
+@pytest.mark.parametrize("defer", [0, 1])
+def test_progress_callback_bit_exact(ctx, defer):
+    # the callback sees committed images (pending commits are flushed first),
+    # and reading them changes nothing
+    ex = synth.example(212, 96, 96)
+    prm = R.default_params(seed=3)
+    job = R.Job(prm, [ex], 192, 160)
+    job.run(threads=8, **DEFAULT)
+    seen = []
+    sess = ts.Session.builder().add_example(ex).seed(3).output_size(ts.Dims(192, 160)).build()
+    sess.set_schedule(**DEFAULT).set_tuning(defer=defer)
+    gen = sess.run(progress=lambda p: seen.append((p["total"][0], int((p["image"][..., 3] > 0).sum()))))
+    assert_same(gen, job)
+    assert len(seen) > 10
+    assert all(a[0] <= b[0] and a[1] <= b[1] for a, b in zip(seen, seen[1:]))
+    assert seen[-1][1] == 192 * 160
 
 
 def test_multi_example_random_init_bit_exact(ctx):

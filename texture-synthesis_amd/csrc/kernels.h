@@ -65,6 +65,35 @@ struct BatchArgs {
     // and the own pattern wrap around the image
     uint64_t* gmask;
     int32_t xl, xr, yb, yt;
+    // deferred commit (k_resolve; null state2: off). Items write their record
+    // to state2 and `serial` to stamp; new pixels also set their bit in
+    // pend_w. Readers take a neighbour from state2 when its stamp is
+    // serial - 1 (the previous batch, which this launch commits) and search
+    // the mask together with pend_r (its new pixels). Each workgroup then
+    // commits a slice of the previous batch [prev0, prev1) into state_w and
+    // mask_w, and clears pend_c for the batch before it [pp0, pp1).
+    uint4* state2;
+    uint32_t* stamp;
+    uint32_t serial;
+    const uint64_t* pend_r;
+    uint64_t* pend_w;
+    uint64_t* pend_c;
+    uint32_t prev0, prev1, pp0, pp1;
+};
+
+// The commit a deferred batch left (its launch was the last of a series):
+// items [prev0, prev1) from state2 into state and mask; pend_prev is cleared
+// for them and pend_pp for items [pp0, pp1).
+struct DeferFlushArgs {
+    uint32_t W, tiles_x;
+    const uint32_t* work;
+    const uint4* state2;
+    uint4* state;
+    uint64_t* mask;
+    uint64_t* pend_prev;
+    uint64_t* pend_pp;
+    uint32_t first_new;
+    uint32_t prev0, prev1, pp0, pp1;
 };
 
 // kNN of stage items [item0, item1) ahead of their batches: item i sees the
@@ -240,6 +269,7 @@ hipError_t launch_resolve_flow(const BatchArgs& a, const FlowArgs& f, uint32_t w
 hipError_t launch_stats(const StatsArgs& a, hipStream_t s);
 hipError_t launch_rand(const RandArgs& a, hipStream_t s);
 hipError_t launch_commit(const CommitArgs& a, hipStream_t s);
+hipError_t launch_defer_flush(const DeferFlushArgs& a, hipStream_t s);
 hipError_t launch_state_snapshot(const uint32_t* rlist, uint32_t n, const uint4* state, uint4* old, hipStream_t s);
 hipError_t launch_recolor(const RecolorArgs& a, hipStream_t s);
 // separable resampling pass: vertical (u8 -> f32) and horizontal (f32 -> u8)

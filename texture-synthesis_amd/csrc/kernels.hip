@@ -230,12 +230,24 @@ struct KnnView {
     uint32_t n_at;
     const uint64_t* gmask;
     int32_t xl, xr, yb, yt;
+    const uint64_t* pend;  // more resolved bits (a deferred commit's new pixels), or null
 };
+
+// The resolved bits of tile ti.
+__device__ __forceinline__ uint64_t mask_word(const KnnView& v, uint32_t ti) {
+    uint64_t w = v.mask[ti];
+    if (v.pend) w |= v.pend[ti];
+    return w;
+}
+
+__device__ __forceinline__ bool view_set(const KnnView& v, int X, int Y) {
+    return (mask_word(v, (uint32_t)(Y >> 3) * v.tiles_x + (uint32_t)(X >> 3)) >> (((Y & 7) << 3) | (X & 7))) & 1ull;
+}
 
 // Whether a point stands at (X, Y), ghosts included (no mask_new).
 __device__ __forceinline__ bool point_at(const KnnView& v, int X, int Y) {
     const bool xin = X >= 0 && X < (int)v.W, yin = Y >= 0 && Y < (int)v.H;
-    if (xin && yin) return px_set(v.mask, v.tiles_x, X, Y);
+    if (xin && yin) return view_set(v, X, Y);
     if (!v.gmask || (!xin && !yin)) return false;
     if (!xin) {
         const int x0 = X >= (int)v.W ? X - (int)v.W : X + (int)v.W;
@@ -268,7 +280,7 @@ __device__ __forceinline__ uint32_t bit_range(int lo, int hi) {
 // may lie partly or wholly outside the image and holds ghost copies there.
 __device__ __forceinline__ uint64_t tile_points(const KnnView& v, int gx, int gy, bool& in_grid) {
     in_grid = gx >= 0 && gy >= 0 && gx < (int)v.tiles_x && gy < (int)v.tiles_y;
-    uint64_t w = in_grid ? v.mask[(uint32_t)gy * v.tiles_x + (uint32_t)gx] : 0ull;
+    uint64_t w = in_grid ? mask_word(v, (uint32_t)gy * v.tiles_x + (uint32_t)gx) : 0ull;
     const int X0 = gx * 8, Y0 = gy * 8, W = (int)v.W, H = (int)v.H;
     if (!v.gmask || (X0 >= 0 && Y0 >= 0 && X0 + 8 <= W && Y0 + 8 <= H)) return w;
     for (int j = 0; j < 8; ++j) {
@@ -479,7 +491,7 @@ __device__ __forceinline__ uint32_t knn_table(const KnnView& a, WaveLds* L, int 
                 o[u] = a.offtab[t];
                 const int X = x + packed_dx(o[u]), Y = y + packed_dy(o[u]);
                 const bool in = X >= 0 && Y >= 0 && X < (int)a.W && Y < (int)a.H;
-                hit[u] = in ? px_set(a.mask, a.tiles_x, X, Y) : point_at(a, X, Y);
+                hit[u] = in ? view_set(a, X, Y) : point_at(a, X, Y);
                 if (in && !hit[u] && a.mask_new && px_set(a.mask_new, a.tiles_x, X, Y))
                     hit[u] = a.rank[(uint32_t)Y * a.W + (uint32_t)X] < a.n_at;
             }
@@ -766,7 +778,7 @@ __device__ __forceinline__ uint32_t item_knn(const BatchArgs& a, uint32_t item, 
         if (lane < (int)kf) L->nd[lane] = a.pre_nbrs[(size_t)(item - a.pre_base) * k + lane];
     } else {
         const KnnView v{a.W,     a.H,     a.tiles_x, a.tiles_y, a.mask, a.rlist, a.offtab, a.offtab_n, nullptr,
-                        nullptr, nullptr, 0u,        a.gmask,   a.xl,   a.xr,    a.yb,     a.yt};
+                        nullptr, nullptr, 0u,        a.gmask,   a.xl,   a.xr,    a.yb,     a.yt,       a.pend_r};
         uint32_t iters, buffered;
         kf = knn_search(v, L, x, y, k, n_res, lane, iters, buffered);
         phase_clock(a, item, 7, lane);
@@ -867,8 +879,16 @@ __device__ __forceinline__ uint32_t item_neighbours(const BatchArgs& a, const Ma
             ny = ny < 0 ? ny + (int)a.H : (ny >= (int)a.H ? ny - (int)a.H : ny);
         }
         uint32_t nflat = (uint32_t)ny * a.W + (uint32_t)nx;
-        const uint4* src = (old_state && !((fresh >> lane) & 1ull)) ? old_state : a.state;
-        uint4 st = src[nflat];
+        uint4 st;
+        if (a.state2) {
+            // deferred commit: the previous batch's records are still in state2
+            const uint32_t sp = a.stamp[nflat];
+            const uint4 s0 = a.state[nflat], s1 = a.state2[nflat];
+            st = sp == a.serial - 1u ? s1 : s0;
+        } else {
+            const uint4* src = (old_state && !((fresh >> lane) & 1ull)) ? old_state : a.state;
+            st = src[nflat];
+        }
         L->pat[lane] = st.x;
         if (GUIDED) L->gpat[lane] = a.tg[nflat];
         if (!a.ext_cands) {
@@ -1317,7 +1337,42 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve(BatchArgs a) {
     const int T = a.team > 1 ? (int)a.team : 1;
     const int team = wave / T, member = wave % T;
     const uint32_t item = a.item0 + blockIdx.x * (uint32_t)(kWavesPerWG / T) + (uint32_t)team;
-    resolve_team<GUIDED>(a, maps, item, a.item0, a.n_res, wls + team * T, member, T, item < a.item1, lut, glut, lane);
+    const uint4 r =
+        resolve_team<GUIDED>(a, maps, item, a.item0, a.n_res, wls + team * T, member, T, item < a.item1, lut, glut, lane);
+    if (a.state2) {
+        if (member == 0 && lane == 0 && item < a.item1) {
+            // the record goes to state2 (Generator::update, ms.rs:334-377);
+            // the score is read only after the run
+            const uint32_t p = a.work[item];
+            const MapDesc& md = maps[r.y & 0xffffu];
+            const uint32_t color = md.px[(r.x >> 16) * md.w + (r.x & 0xffffu)];
+            a.state2[p] = make_uint4(color, r.x, r.y & ~kFallbackBit, r.z);
+            a.stamp[p] = a.serial;
+            if (item >= a.first_new) {
+                const uint32_t px = p % a.W, py = p / a.W;
+                atomicOr((unsigned long long*)&a.pend_w[(py >> 3) * a.tiles_x + (px >> 3)],
+                         1ull << (((py & 7u) << 3) | (px & 7u)));
+                a.score_w[p] = __uint_as_float(r.w);
+            }
+        }
+        // the previous batch's commit, and the pending bits of the one before
+        const uint32_t stride = gridDim.x * blockDim.x, t0 = blockIdx.x * blockDim.x + threadIdx.x;
+        for (uint32_t i = a.prev0 + t0; i < a.prev1; i += stride) {
+            const uint32_t p = a.work[i];
+            a.state_w[p] = a.state2[p];
+            if (i >= a.first_new) {
+                const uint32_t px = p % a.W, py = p / a.W;
+                atomicOr((unsigned long long*)&a.mask_w[(py >> 3) * a.tiles_x + (px >> 3)],
+                         1ull << (((py & 7u) << 3) | (px & 7u)));
+            }
+        }
+        for (uint32_t i = a.pp0 + t0; i < a.pp1; i += stride) {
+            const uint32_t p = a.work[i];
+            a.pend_c[((p / a.W) >> 3) * a.tiles_x + ((p % a.W) >> 3)] = 0ull;
+        }
+        launch_span(a, t_start);
+        return;
+    }
     if (!a.fused_commit) {
         launch_span(a, t_start);
         return;
@@ -1661,7 +1716,25 @@ __global__ void k_commit(CommitArgs a) {
                a.gmask);
 }
 
-// next_pyramid_level (ms.rs:687-700): recolour every resolved pixel.
+// DeferFlushArgs: commit the last deferred batch, clear the pending bits.
+__global__ void k_defer_flush(DeferFlushArgs a) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = a.prev0 + t;
+    if (i < a.prev1) {
+        const uint32_t p = a.work[i];
+        a.state[p] = a.state2[p];
+        const uint32_t px = p % a.W, py = p / a.W;
+        const uint32_t ti = (py >> 3) * a.tiles_x + (px >> 3);
+        if (i >= a.first_new) atomicOr((unsigned long long*)&a.mask[ti], 1ull << (((py & 7u) << 3) | (px & 7u)));
+        a.pend_prev[ti] = 0ull;
+    }
+    const uint32_t q = a.pp0 + t;
+    if (q < a.pp1) {
+        const uint32_t p = a.work[q];
+        a.pend_pp[((p / a.W) >> 3) * a.tiles_x + ((p % a.W) >> 3)] = 0ull;
+    }
+}
+
 // old[p] = state[p] for the first n pixels of the resolved list.
 __global__ void k_state_snapshot(const uint32_t* rlist, uint32_t n, const uint4* state, uint4* old) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1670,6 +1743,7 @@ __global__ void k_state_snapshot(const uint32_t* rlist, uint32_t n, const uint4*
     old[p] = state[p];
 }
 
+// next_pyramid_level (ms.rs:687-700): recolour every resolved pixel.
 __global__ void k_recolor(RecolorArgs a) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
@@ -2005,6 +2079,13 @@ hipError_t launch_commit(const CommitArgs& a, hipStream_t s) {
     uint32_t n = a.item1 - a.item0;
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_commit, dim3((n + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_defer_flush(const DeferFlushArgs& a, hipStream_t s) {
+    const uint32_t n0 = a.prev1 - a.prev0, n1 = a.pp1 - a.pp0, n = n0 > n1 ? n0 : n1;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_defer_flush, dim3((n + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -84,6 +84,7 @@ class Tuning(C.Structure):
         ("flow", C.c_uint32),
         ("flow_wgs", C.c_uint32),
         ("flow_team", C.c_uint32),
+        ("defer", C.c_uint32),
     ]
 
 

@@ -27,6 +27,7 @@ SPACE: Sequence[Tuple[str, Sequence[int]]] = (
     ("small_max", (0, 4, 16)),
     ("prefetch_max", (1024, 2048, 4096)),
     ("ahead", (1, 2)),
+    ("defer", (1, 0)),
 )
 
 
