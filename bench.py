@@ -8,8 +8,9 @@ reference's criterion benches build outside the timer
 (lib/benches/all-the-things.rs:20-29), so inputs are resident in HBM.
 
 Prints one JSON line: Mpix/s of output, per-step latency p50/p99, the
-roofline of the dominant kernel (k_resolve) from HIP events on the
-library's stream, and the CPU oracle timed on the host's cores.
+roofline of the dominant kernel (the throughput-bound batch launches:
+k_resolve_persist for single-wave items, k_resolve for teams) from the
+launches' device-clock spans, and the CPU oracle timed on the host's cores.
 
 Two more legs report in the same line:
 * "batched": 256 independent syntheses (examples uniform in {64..512}^2,
@@ -216,8 +217,9 @@ def serving_leg(ts, synth, n, qps, budget_gib):
 
 
 def traffic_from_profiles():
-    """HBM bytes per k_resolve launch of this workload, from the committed
-    counter summary (FETCH_SIZE + WRITE_SIZE per dispatch, copy-calibrated)."""
+    """HBM bytes per batch launch (k_resolve_persist and k_resolve) of this
+    workload, from the committed counter summary (FETCH_SIZE + WRITE_SIZE per
+    dispatch, copy-calibrated)."""
     import re
     path = os.path.join(ROOT, "profiles", "resolve_c2_traffic.md")
     if not os.path.exists(path):
@@ -303,9 +305,10 @@ def main():
     # profiled run (separate, after the timed region): per-kernel device time
     g = prof.run()
     ks = g.kernel_stats
-    # the dominant kernel: k_resolve, one launch per batch of more than a few
-    # items (the one-workgroup loop over the smallest batches is reported apart);
-    # durations come from the dispatches' own start/end timestamps
+    # the dominant kernel: one launch per batch of more than a few items
+    # (k_resolve_persist, or k_resolve for teams; the one-workgroup loop over
+    # the smallest batches is reported apart); durations come from the
+    # launches' device-clock spans
     res = ks["resolve"]
     resolves = res["items"] + ks["resolve_runs"]["items"]
     kern_s = res["ms"] / 1e3
@@ -340,7 +343,7 @@ def main():
                                    "resolved list downloaded to host memory"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
-                     "kernel": "k_resolve", "bytes_per_resolve": BYTES_PER_RESOLVE,
+                     "kernel": "k_resolve_persist + k_resolve", "bytes_per_resolve": BYTES_PER_RESOLVE,
                      "algorithmic_bytes_per_launch": round(per_launch),
                      "avg_launch_ms": round(res["ms"] / max(1, res["launches"]), 4),
                      "launches": res["launches"]},

@@ -109,7 +109,11 @@ typedef struct tsg_schedule {
  *                done by the next such batch's launch (its items read the
  *                previous batch's records from a second state array), so a
  *                series of batches needs one launch each instead of two;
- *                0 (default): a commit launch after every batch */
+ *                0 (default): a commit launch after every batch
+ *   persist      1: a batch of single-wave items runs on resident
+ *                workgroups, each over an equal slice whose items its waves
+ *                take in turn (no wave waits for the slowest item of its
+ *                workgroup); 0: one workgroup per 8 items */
 typedef struct tsg_tuning {
     uint32_t small_max;
     uint32_t fuse_max;
@@ -123,6 +127,7 @@ typedef struct tsg_tuning {
     uint32_t flow_wgs;
     uint32_t flow_team;
     uint32_t defer;
+    uint32_t persist;
 } tsg_tuning;
 
 typedef struct tsg_image {

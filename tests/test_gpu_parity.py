@@ -254,6 +254,9 @@ def test_default_schedule_bit_exact_c1(ctx, seed):
     dict(defer=1, flow=0, ahead=0, prefetch_max=0, small_max=0, team_max=1),
     dict(defer=1, flow=0, ahead=0, prefetch_max=64, small_max=16, fuse_max=64),
     dict(defer=1, coop_wgs=8, coop_max=128, ahead=0),
+    dict(persist=0),
+    dict(persist=1, team_max=1, ahead=0, flow=0, prefetch_max=0, small_max=0),
+    dict(persist=1, team_waves=64),
 ])
 def test_launch_tuning_does_not_change_results(ctx, tuning):
     # every launch structure resolves the same batches against the same states

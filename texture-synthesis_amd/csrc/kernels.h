@@ -269,6 +269,9 @@ hipError_t launch_resolve_flow(const BatchArgs& a, const FlowArgs& f, uint32_t w
 hipError_t launch_stats(const StatsArgs& a, hipStream_t s);
 hipError_t launch_rand(const RandArgs& a, hipStream_t s);
 hipError_t launch_commit(const CommitArgs& a, hipStream_t s);
+// Single-wave batches on persistent workgroups; persist_max_wgs: resident count.
+uint32_t persist_max_wgs(bool guided);
+hipError_t launch_resolve_persist(const BatchArgs& a, uint32_t wgs, bool guided, hipStream_t s);
 hipError_t launch_defer_flush(const DeferFlushArgs& a, hipStream_t s);
 hipError_t launch_state_snapshot(const uint32_t* rlist, uint32_t n, const uint4* state, uint4* old, hipStream_t s);
 hipError_t launch_recolor(const RecolorArgs& a, hipStream_t s);

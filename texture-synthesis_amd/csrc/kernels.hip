@@ -1391,6 +1391,46 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve(BatchArgs a) {
     launch_span(a, t_start);
 }
 
+// One batch of single-wave items on persistent workgroups (at most as many
+// as are resident): workgroup w takes an equal slice of the batch, and its
+// waves take the slice's items from an LDS counter until they run out, so a
+// wave never waits for the slowest item of its workgroup, and the cost
+// tables are set up once per workgroup. (One device-wide counter for every
+// wave serialised the item handout.) Results go to a.result as in k_resolve.
+template <bool GUIDED>
+__global__ __launch_bounds__(kWave* kWavesPerWG) __attribute__((amdgpu_waves_per_eu(4))) void k_resolve_persist(
+    BatchArgs a) {
+    const uint64_t t_start = a.dbg_launch ? wall_clock64() : 0;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ MapDesc smaps[kMapCache];
+    __shared__ uint32_t s_next;
+    if (threadIdx.x == 0) s_next = 0;
+    float *lut, *glut;
+    const MapDesc* maps;
+    WaveLds* wls = setup_lds<GUIDED>(a, smem, lut, glut, smaps, maps);  // ends with a barrier
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    const uint32_t n = a.item1 - a.item0;
+    const uint32_t s0 = (uint32_t)((uint64_t)blockIdx.x * n / gridDim.x);
+    const uint32_t s1 = (uint32_t)((uint64_t)(blockIdx.x + 1) * n / gridDim.x);
+    // the launcher admits a.team <= 1 only; kept a runtime value, as in k_resolve
+    const int T = a.team > 1 ? (int)a.team : 1;
+    const int member = wave % T;
+    for (;;) {
+        uint32_t idx = 0;
+        if (lane == 0) idx = s0 + atomicAdd(&s_next, 1u);
+        idx = (uint32_t)__shfl((int)idx, 0);
+        if (idx >= s1) break;
+        // an opaque lane index: the item's lane-dependent addresses are
+        // computed per item instead of held in registers across the loop
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
+        resolve_team<GUIDED>(a, maps, a.item0 + idx, a.item0, a.n_res, wls + (wave - member), member, T, true, lut,
+                             glut, ln);
+    }
+    launch_span(a, t_start);
+}
+
 // A run of small consecutive batches (at most kSmallWaves items each) in one
 // workgroup: resolve, barrier, commit, barrier, next batch. Batch b covers
 // items [bounds[b], bounds[b+1]) with nres[b] resolved pixels at its start.
@@ -1840,6 +1880,12 @@ hipError_t configure_kernels() {
     hipError_t e = hipFuncSetAttribute((const void*)k_resolve<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)resolve_lds_bytes(true));
     if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)k_resolve_persist<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)resolve_lds_bytes(true));
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)k_resolve_persist<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)resolve_lds_bytes(false));
+    if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)k_resolve<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)resolve_lds_bytes(false));
     if (e != hipSuccess) return e;
@@ -2011,6 +2057,31 @@ hipError_t launch_resolve_small(const BatchArgs& a, const uint32_t* bounds, cons
 
 size_t flow_lds_bytes(uint32_t team, bool guided) {
     return (guided ? 2 : 1) * 256 * kLutCopies * sizeof(float) + team * sizeof(WaveLds);
+}
+
+uint32_t persist_max_wgs(bool guided) {
+    int per_cu = 0, cus = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    const void* f = guided ? (const void*)k_resolve_persist<true> : (const void*)k_resolve_persist<false>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kWave * kWavesPerWG, resolve_lds_bytes(guided)) !=
+        hipSuccess)
+        return 0;
+    return (uint32_t)(per_cu * cus);
+}
+
+hipError_t launch_resolve_persist(const BatchArgs& a, uint32_t wgs, bool guided, hipStream_t s) {
+    if (a.item1 <= a.item0) return hipSuccess;
+    if (wgs == 0 || a.team > 1 || a.fused_commit || a.state2) return hipErrorInvalidValue;
+    const uint32_t lds = (uint32_t)resolve_lds_bytes(guided);
+    if (guided) {
+        hipExtLaunchKernelGGL(k_resolve_persist<true>, dim3(wgs), dim3(kWave * kWavesPerWG), lds, s, g_ev_start,
+                              g_ev_stop, 0u, a);
+    } else {
+        hipExtLaunchKernelGGL(k_resolve_persist<false>, dim3(wgs), dim3(kWave * kWavesPerWG), lds, s, g_ev_start,
+                              g_ev_stop, 0u, a);
+    }
+    return hipGetLastError();
 }
 
 uint32_t flow_max_wgs(uint32_t team, bool guided) {

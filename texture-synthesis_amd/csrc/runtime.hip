@@ -1503,12 +1503,15 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
             HIP_TRY(hipMemsetAsync(j->flow_flag.p, 0, j->N * 4, s));
             j->flow_epoch = 0;
         }
-        if (!j->flow_ticket.n) {
-            HIP_TRY(j->flow_ticket.alloc(1));
-            HIP_TRY(hipMemsetAsync(j->flow_ticket.p, 0, 4, s));
-            j->flow_tickets = 0;
-        }
         if (j->tune.flow >= 2 && j->state_old.n < j->N) HIP_TRY(j->state_old.alloc(j->N));
+    }
+    // persistent single-wave batches
+    const uint32_t persist_wgs = j->tune.persist ? persist_max_wgs(j->guided) : 0u;
+    if (flow_on && !j->flow_ticket.n) {
+        // the ticket counter of dependency-ordered runs
+        HIP_TRY(j->flow_ticket.alloc(1));
+        HIP_TRY(hipMemsetAsync(j->flow_ticket.p, 0, 4, s));
+        j->flow_tickets = 0;
     }
     // deferred commits for throughput-bound batches (not in tiling mode,
     // whose commits also set ghost flags)
@@ -1696,7 +1699,13 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
                     continue;
                 }
                 if ((rc = flush_def())) return rc;
-                HIP_TRY(timed_launch(j, K_RESOLVE, i1 - i0, [&] { return launch_resolve(a, j->guided, s); }));
+                if (persist_wgs && a.team <= 1 && !a.fused_commit) {
+                    const uint32_t g = (uint32_t)std::min<uint64_t>(persist_wgs, (i1 - i0 + 7) / 8);
+                    HIP_TRY(timed_launch(j, K_RESOLVE, i1 - i0,
+                                         [&] { return launch_resolve_persist(a, g, j->guided, s); }));
+                } else {
+                    HIP_TRY(timed_launch(j, K_RESOLVE, i1 - i0, [&] { return launch_resolve(a, j->guided, s); }));
+                }
                 if (!a.fused_commit) {
                     CommitArgs ca{j->W,       j->tiles_x, a.maps,    a.work,      j->result.p, (uint32_t)i0, (uint32_t)i1,
                                   a.first_new, j->state.p, j->mask.p, j->score.p, a.gmask};
@@ -1734,6 +1743,7 @@ int check_tuning(const tsg_tuning* t) {
         return fail(TSG_ERR_INVALID_ARGUMENT, "coop_max above 16 items per cooperative workgroup");
     if (t->flow > 2) return fail(TSG_ERR_INVALID_ARGUMENT, "flow must be 0, 1 or 2");
     if (t->defer > 1) return fail(TSG_ERR_INVALID_ARGUMENT, "defer must be 0 or 1");
+    if (t->persist > 1) return fail(TSG_ERR_INVALID_ARGUMENT, "persist must be 0 or 1");
     if (t->flow && (t->flow_team == 0 || t->flow_team > 8 || (t->flow_team & (t->flow_team - 1))))
         return fail(TSG_ERR_INVALID_ARGUMENT, "flow_team must be 1, 2, 4 or 8");
     if (t->ahead > 2) return fail(TSG_ERR_INVALID_ARGUMENT, "ahead must be 0, 1 or 2");
@@ -2269,6 +2279,7 @@ void tsg_tuning_default(tsg_tuning* t) {
     // (2048^2), 3% faster at 256^2 and 512^2: it saves the commit dispatch
     // (5.4 us) but each resolve dispatch grows by 7.6 us
     t->defer = 0;
+    t->persist = 1;
 }
 
 int tsg_job_set_tuning(tsg_job* job, const tsg_tuning* t) {

@@ -85,6 +85,7 @@ class Tuning(C.Structure):
         ("flow_wgs", C.c_uint32),
         ("flow_team", C.c_uint32),
         ("defer", C.c_uint32),
+        ("persist", C.c_uint32),
     ]
 
 
