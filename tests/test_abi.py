@@ -57,3 +57,31 @@ def test_autotune_space_names_tuning_knobs():
     t = tsgpu.default_tuning()
     for knob, values in SPACE:
         assert knob in t and len(values) >= 2
+
+
+def header_struct_fields(name):
+    """Field names of `typedef struct name { ... } name;` in include/tsgpu.h."""
+    with open(os.path.join(ROOT, "include", "tsgpu.h")) as f:
+        text = re.sub(r"/\*.*?\*/", "", f.read(), flags=re.S)
+    m = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), text, re.S)
+    assert m, name
+    fields = []
+    for decl in m.group(1).split(";"):
+        decl = re.sub(r"//.*", "", decl).strip()
+        if decl:
+            fields.append(re.findall(r"(\w+)\s*(?:\[[^]]*\])?$", decl)[0])
+    return fields
+
+
+def test_python_mirrors_match_the_header_structs():
+    # the ctypes mirrors must keep the header's field order (a drift would
+    # shift every later field across the ABI)
+    for cname, py in (("tsg_tuning", tsgpu.Tuning), ("tsg_schedule", tsgpu.Schedule), ("tsg_params", tsgpu.Params)):
+        assert [n for n, _ in py._fields_] == header_struct_fields(cname), cname
+
+
+def test_tuning_defaults():
+    # every knob is valid, and the measured defaults are the documented ones
+    t = tsgpu.default_tuning()
+    assert t["flow"] == 1 and t["defer"] == 0 and t["persist"] == 1 and t["ahead"] == 1
+    assert t["flow_team"] in (1, 2, 4, 8) and t["team_max"] in (1, 2, 4, 8)
