@@ -113,7 +113,11 @@ typedef struct tsg_schedule {
  *   persist      1: a batch of single-wave items runs on resident
  *                workgroups, each over an equal slice whose items its waves
  *                take in turn (no wave waits for the slowest item of its
- *                workgroup); 0: one workgroup per 8 items */
+ *                workgroup); 0: one workgroup per 8 items
+ *   flow_head    leading items of each dependency-ordered run that go to a
+ *                launch of their own, with teams of flow_head_team waves
+ *                (the run's first items form its longest serial chain)
+ *   flow_head_team  waves per item in that launch (1, 2, 4 or 8) */
 typedef struct tsg_tuning {
     uint32_t small_max;
     uint32_t fuse_max;
@@ -128,6 +132,8 @@ typedef struct tsg_tuning {
     uint32_t flow_team;
     uint32_t defer;
     uint32_t persist;
+    uint32_t flow_head;
+    uint32_t flow_head_team;
 } tsg_tuning;
 
 typedef struct tsg_image {

@@ -257,6 +257,9 @@ def test_default_schedule_bit_exact_c1(ctx, seed):
     dict(persist=0),
     dict(persist=1, team_max=1, ahead=0, flow=0, prefetch_max=0, small_max=0),
     dict(persist=1, team_waves=64),
+    dict(flow_head=0),
+    dict(flow_head=64, flow_head_team=2, flow_wgs=5),
+    dict(flow=2, flow_head=100, flow_head_team=8),
 ])
 def test_launch_tuning_does_not_change_results(ctx, tuning):
     # every launch structure resolves the same batches against the same states

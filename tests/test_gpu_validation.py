@@ -102,7 +102,8 @@ def test_tiny_ragged_example_bit_exact(ctx):
     assert_same(gen, job)
 
 
-@pytest.mark.parametrize("knob,value", [("flow", 3), ("flow_team", 3), ("ahead", 3), ("team_max", 16)])
+@pytest.mark.parametrize("knob,value", [("flow", 3), ("flow_team", 3), ("ahead", 3), ("team_max", 16),
+                                        ("defer", 2), ("persist", 2), ("flow_head_team", 16)])
 def test_tuning_out_of_range(ctx, knob, value):
     s = builder().build()
     with pytest.raises(ts.TsgError) as e:

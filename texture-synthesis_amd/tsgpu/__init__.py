@@ -86,6 +86,8 @@ class Tuning(C.Structure):
         ("flow_team", C.c_uint32),
         ("defer", C.c_uint32),
         ("persist", C.c_uint32),
+        ("flow_head", C.c_uint32),
+        ("flow_head_team", C.c_uint32),
     ]
 
 

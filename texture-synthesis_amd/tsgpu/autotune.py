@@ -29,6 +29,7 @@ SPACE: Sequence[Tuple[str, Sequence[int]]] = (
     ("ahead", (1, 2)),
     ("defer", (1, 0)),
     ("persist", (1, 0)),
+    ("flow_head", (1024, 0, 4096)),
 )
 
 
