@@ -231,7 +231,17 @@ struct KnnView {
     const uint64_t* gmask;
     int32_t xl, xr, yb, yt;
     const uint64_t* pend;  // more resolved bits (a deferred commit's new pixels), or null
+    const int32_t* offc;   // the table's first kOffCache offsets in LDS, or null
 };
+
+constexpr uint32_t kOffCache = 256;  // one table round
+
+// The workgroup's LDS copy of the offset table's first round (filled by
+// setup_lds in every resolve kernel).
+__device__ __forceinline__ int32_t* offtab_lds() {
+    __shared__ int32_t cache[kOffCache];
+    return cache;
+}
 
 // The resolved bits of tile ti.
 __device__ __forceinline__ uint64_t mask_word(const KnnView& v, uint32_t ti) {
@@ -488,7 +498,7 @@ __device__ __forceinline__ uint32_t knn_table(const KnnView& a, WaveLds* L, int 
             o[u] = 0;
             hit[u] = false;
             if (t < a.offtab_n) {
-                o[u] = a.offtab[t];
+                o[u] = (a.offc && base < kOffCache) ? a.offc[t] : a.offtab[t];
                 const int X = x + packed_dx(o[u]), Y = y + packed_dy(o[u]);
                 const bool in = X >= 0 && Y >= 0 && X < (int)a.W && Y < (int)a.H;
                 hit[u] = in ? view_set(a, X, Y) : point_at(a, X, Y);
@@ -778,7 +788,8 @@ __device__ __forceinline__ uint32_t item_knn(const BatchArgs& a, uint32_t item, 
         if (lane < (int)kf) L->nd[lane] = a.pre_nbrs[(size_t)(item - a.pre_base) * k + lane];
     } else {
         const KnnView v{a.W,     a.H,     a.tiles_x, a.tiles_y, a.mask, a.rlist, a.offtab, a.offtab_n, nullptr,
-                        nullptr, nullptr, 0u,        a.gmask,   a.xl,   a.xr,    a.yb,     a.yt,       a.pend_r};
+                        nullptr, nullptr, 0u,        a.gmask,   a.xl,   a.xr,    a.yb,     a.yt,       a.pend_r,
+                        offtab_lds()};
         uint32_t iters, buffered;
         kf = knn_search(v, L, x, y, k, n_res, lane, iters, buffered);
         phase_clock(a, item, 7, lane);
@@ -1305,6 +1316,8 @@ __device__ __forceinline__ WaveLds* setup_lds(const BatchArgs& a, unsigned char*
         if (threadIdx.x < a.n_maps) smaps[threadIdx.x] = a.maps[threadIdx.x];
         maps = smaps;
     }
+    int32_t* oc = offtab_lds();
+    for (uint32_t t = threadIdx.x; t < kOffCache && t < a.offtab_n; t += blockDim.x) oc[t] = a.offtab[t];
     __syncthreads();
     return reinterpret_cast<WaveLds*>(smem + (GUIDED ? 2 : 1) * 256 * kLutCopies * sizeof(float));
 }
