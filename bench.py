@@ -10,7 +10,8 @@ reference's criterion benches build outside the timer
 Prints one JSON line: Mpix/s of output, per-step latency p50/p99, the
 roofline of the dominant kernel (the throughput-bound batch launches:
 k_resolve_persist for single-wave items, k_resolve for teams) from the
-launches' device-clock spans, and the CPU oracle timed on the host's cores.
+launches' device-clock spans in the timed steps, and the CPU oracle timed on
+the host's cores.
 
 Two more legs report in the same line:
 * "batched": 256 independent syntheses (examples uniform in {64..512}^2,
@@ -45,10 +46,10 @@ BYTES_PER_RESOLVE = 4 * K * (K + M) + 24 * K + 28
 WORKLOAD = "plain: 512x512 RGBA8 sinusoid+noise example (seed 512) -> 1024x1024, defaults (k=50, m=50, P=5, p=0.5)"
 
 
-def build_session(ts, ex, profile=False):
+def build_session(ts, ex, profile=0):
     sess = ts.Session.builder().add_example(ex).seed(0).output_size(ts.Dims.square(1024)).build()
     if profile:
-        sess.set_profiling(True)
+        sess.set_profiling(profile)
     return sess
 
 
@@ -285,32 +286,35 @@ def main():
         from tsgpu.autotune import autotune
         b = ts.Session.builder().add_example(ex).seed(0).output_size(ts.Dims.square(1024))
         tuned = autotune(b, reps=2, ctx=ctx)
-    sessions = [build_session(ts, ex) for _ in range(args.steps)]
-    prof = build_session(ts, ex, profile=True)
+    # the timed steps record the resolve kernels' device-clock spans (no events)
+    sessions = [build_session(ts, ex, profile=-1) for _ in range(args.steps)]
+    prof = build_session(ts, ex, profile=1)
     # a step: the synthesis with its output left in device memory, then the
     # job's release; the download to host memory (PCIe) is timed apart
     lat, lat_pcie = [], []
+    res = {"ms": 0.0, "items": 0, "launches": 0}  # batch launches of the timed steps
     for s in sessions:
         t0 = time.perf_counter()
         s.run_resident()
         t1 = time.perf_counter()
-        s.download()
+        step_res = s.download().kernel_stats["resolve"]
         t2 = time.perf_counter()
+        for key in res:
+            res[key] += step_res[key]
         s.close()
         t3 = time.perf_counter()
         lat.append(((t1 - t0) + (t3 - t2)) * 1e3)
         lat_pcie.append((t3 - t0) * 1e3)
     t_all = sum(lat) / 1e3
     t_pcie = sum(lat_pcie) / 1e3
-    # profiled run (separate, after the timed region): per-kernel device time
+    # profiled run (separate, after the timed region): every kernel's device time
     g = prof.run()
     ks = g.kernel_stats
     # the dominant kernel: one launch per batch of more than a few items
     # (k_resolve_persist, or k_resolve for teams; the one-workgroup loop over
-    # the smallest batches is reported apart); durations come from the
-    # launches' device-clock spans
-    res = ks["resolve"]
-    resolves = res["items"] + ks["resolve_runs"]["items"]
+    # the smallest batches is reported apart), over the timed steps; durations
+    # are the launches' device-clock spans
+    resolves = (res["items"] + ks["resolve_runs"]["items"] * args.steps) // args.steps
     kern_s = res["ms"] / 1e3
     achieved = res["items"] * BYTES_PER_RESOLVE / kern_s / 1e9 if kern_s > 0 else 0.0
     per_launch = res["items"] * BYTES_PER_RESOLVE / max(1, res["launches"])
@@ -346,7 +350,7 @@ def main():
                      "kernel": "k_resolve_persist + k_resolve", "bytes_per_resolve": BYTES_PER_RESOLVE,
                      "algorithmic_bytes_per_launch": round(per_launch),
                      "avg_launch_ms": round(res["ms"] / max(1, res["launches"]), 4),
-                     "launches": res["launches"]},
+                     "launches": res["launches"], "over": "the timed steps"},
         "kernels_ms": {k: round(v["ms"], 3) for k, v in ks.items()},
         "device_ms_per_step": round(g.run_ms, 3),
     }

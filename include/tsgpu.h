@@ -225,7 +225,9 @@ int tsg_job_stage_stats(tsg_job* job, double* sums, uint64_t* counts, uint64_t* 
 /* Device time of the last tsg_job_run in milliseconds, and batch count. */
 int tsg_job_timing(tsg_job* job, double* run_ms, uint64_t* n_batches);
 /* Per-kernel device timing of the next tsg_job_run (HIP events on the job's
- * stream around every launch). Level L >= 2 also records, for each work item
+ * stream around every launch). Level -1 records only the resolve kernels'
+ * execution spans on the device clock (no events; light enough for timed
+ * runs: the other kernels then report nothing). Level L >= 2 also records, for each work item
  * of stage L - 2 (0 = the first), device timestamps (100 MHz wall clock) at
  * the phase boundaries of its resolve: start, kNN, neighbour state and
  * coherence candidates, first candidate group, all groups, end. */

@@ -1323,12 +1323,15 @@ __device__ __forceinline__ WaveLds* setup_lds(const BatchArgs& a, unsigned char*
 }
 
 // Profiling: the launch's execution span on the device clock, as the
-// earliest workgroup start and the latest workgroup end.
+// earliest workgroup start (kept bit-inverted, so that both words take a
+// maximum and zero means empty) and the latest workgroup end. Workgroups are
+// dispatched in index order, so the first eight carry the earliest start;
+// leaving the others out halves the atomics on the two words.
 __device__ __forceinline__ void launch_span(const BatchArgs& a, uint64_t t_start) {
     if (!a.dbg_launch) return;
     __syncthreads();
     if (threadIdx.x == 0) {
-        atomicMin(&a.dbg_launch[0], (unsigned long long)t_start);
+        if (blockIdx.x < 8) atomicMax(&a.dbg_launch[0], ~(unsigned long long)t_start);
         atomicMax(&a.dbg_launch[1], (unsigned long long)wall_clock64());
     }
 }

@@ -876,6 +876,17 @@ hipError_t timed_launch_on(tsg_job* j, hipStream_t s, int kind, uint64_t items, 
     if (!j->profile) return launch();
     tsg_job::Timed t{kind, items, nullptr, nullptr, -1};
     hipError_t e;
+    if (j->profile_level < 0) {
+        // spans only: the resolve kernels' device-clock spans, no events
+        if ((kind != K_RESOLVE && kind != K_RUN) || j->launch_slots >= j->launch_clk.n / 2) return launch();
+        t.slot = (int64_t)j->launch_slots++;
+        j->step.a.dbg_launch = j->launch_clk.p + 2 * t.slot;
+        e = launch();
+        j->step.a.dbg_launch = nullptr;
+        if (e != hipSuccess) return e;
+        j->timed.push_back(t);
+        return hipSuccess;
+    }
     if ((e = next_event(j, &t.a)) != hipSuccess) return e;
     if ((e = next_event(j, &t.b)) != hipSuccess) return e;
     if ((kind == K_RESOLVE || kind == K_RUN) && j->launch_slots < j->launch_clk.n / 2) {
@@ -1111,11 +1122,9 @@ int step_begin(tsg_job* j) {
     j->ev_next = 0;
     j->launch_slots = 0;
     if (j->profile) {
-        // spans start empty: {UINT64_MAX, 0}
-        HIP_TRY(j->launch_clk.reserve(2 * 65536));
-        std::vector<unsigned long long> init(j->launch_clk.n);
-        for (size_t q = 0; q < init.size(); q += 2) init[q] = ~0ull, init[q + 1] = 0;
-        HIP_TRY(hipMemcpy(j->launch_clk.p, init.data(), init.size() * 8, hipMemcpyHostToDevice));
+        // spans start empty: {~start, end} = {0, 0}; 8192 timed launches
+        HIP_TRY(j->launch_clk.reserve(2 * 8192));
+        HIP_TRY(hipMemsetAsync(j->launch_clk.p, 0, j->launch_clk.n * 8, s));
     }
     for (int q = 0; q < kKinds; ++q) j->kern_ms[q] = 0.0, j->kern_items[q] = 0, j->kern_launches[q] = 0;
     const int P = (int)j->prm.backtrack_stages;
@@ -1433,9 +1442,9 @@ int step_finish(tsg_job* j) {
         HIP_TRY(hipMemcpy(spans.data(), j->launch_clk.p, spans.size() * 8, hipMemcpyDeviceToHost));
     for (const tsg_job::Timed& t : j->timed) {
         float km = 0.0f;
-        HIP_TRY(hipEventElapsedTime(&km, t.a, t.b));
-        if (t.slot >= 0) {
-            const unsigned long long a0 = spans[2 * t.slot], a1 = spans[2 * t.slot + 1];
+        if (t.a) HIP_TRY(hipEventElapsedTime(&km, t.a, t.b));
+        if (t.slot >= 0 && spans[2 * t.slot] != 0) {
+            const unsigned long long a0 = ~spans[2 * t.slot], a1 = spans[2 * t.slot + 1];
             if (a0 <= a1) km = (float)((double)(a1 - a0) * 1e-5);  // 100 MHz device clock, in ms
         }
         j->kern_ms[t.kind] += km;
@@ -1708,7 +1717,10 @@ int run_job(tsg_job* j, tsg_progress_fn cb, void* user) {
                         da.pp0 = (uint32_t)dq0;
                         da.pp1 = (uint32_t)dq1;
                     }
-                    HIP_TRY(timed_launch(j, K_RESOLVE, i1 - i0, [&] { return launch_resolve(da, j->guided, s); }));
+                    HIP_TRY(timed_launch(j, K_RESOLVE, i1 - i0, [&] {
+                        da.dbg_launch = a.dbg_launch;  // set by the profiler for this launch
+                        return launch_resolve(da, j->guided, s);
+                    }));
                     dq0 = d_pending ? dp0 : 0;
                     dq1 = d_pending ? dp1 : 0;
                     dp0 = i0;
