@@ -1424,7 +1424,9 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) __attribute__((amdgpu_waves_per
     float *lut, *glut;
     const MapDesc* maps;
     WaveLds* wls = setup_lds<GUIDED>(a, smem, lut, glut, smaps, maps);  // ends with a barrier
-    const int wave = threadIdx.x / kWave;
+    // wave-uniform in a scalar register: the wave's LDS block is then a
+    // scalar base, not a vector address held across the item loop
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
     const int lane = threadIdx.x % kWave;
     const uint32_t n = a.item1 - a.item0;
     const uint32_t s0 = (uint32_t)((uint64_t)blockIdx.x * n / gridDim.x);
