@@ -1348,7 +1348,7 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve(BatchArgs a) {
     float *lut, *glut;
     const MapDesc* maps;
     WaveLds* wls = setup_lds<GUIDED>(a, smem, lut, glut, smaps, maps);
-    const int wave = threadIdx.x / kWave;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));  // scalar LDS base
     const int lane = threadIdx.x % kWave;
     const int T = a.team > 1 ? (int)a.team : 1;
     const int team = wave / T, member = wave % T;
