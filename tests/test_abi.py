@@ -83,6 +83,6 @@ def test_python_mirrors_match_the_header_structs():
 def test_tuning_defaults():
     # every knob is valid, and the measured defaults are the documented ones
     t = tsgpu.default_tuning()
-    assert t["flow"] == 1 and t["defer"] == 0 and t["persist"] == 1 and t["ahead"] == 1
+    assert t["flow"] == 1 and t["defer"] == 0 and t["persist"] == 0 and t["ahead"] == 1
     assert t["flow_head"] == 1024 and t["flow_head_team"] == 8
     assert t["flow_team"] in (1, 2, 4, 8) and t["team_max"] in (1, 2, 4, 8)

@@ -254,7 +254,7 @@ def test_default_schedule_bit_exact_c1(ctx, seed):
     dict(defer=1, flow=0, ahead=0, prefetch_max=0, small_max=0, team_max=1),
     dict(defer=1, flow=0, ahead=0, prefetch_max=64, small_max=16, fuse_max=64),
     dict(defer=1, coop_wgs=8, coop_max=128, ahead=0),
-    dict(persist=0),
+    dict(persist=1),
     dict(persist=1, team_max=1, ahead=0, flow=0, prefetch_max=0, small_max=0),
     dict(persist=1, team_waves=64),
     dict(flow_head=0),

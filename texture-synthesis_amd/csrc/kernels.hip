@@ -29,9 +29,10 @@ constexpr int kWavesPerWG = 8;
 constexpr int kSmallWaves = 16;  // items per batch of k_resolve_small
 constexpr int kKeyCap = 256;
 // Interleaved copies of each 256-entry cost table in LDS (lane l reads copy
-// l % kLutCopies). 16 copies keep two 8-wave workgroups resident per CU with
-// both tables (guided jobs); 32 fitted one, at 26% more guided device time.
-constexpr int kLutCopies = 16;
+// l % kLutCopies). With 8 copies an unguided 8-wave workgroup needs 51 KB, so
+// three are resident per CU (24 waves); a guided one (two tables) 59 KB, two
+// per CU. 32 copies fitted a single guided workgroup per CU.
+constexpr int kLutCopies = 8;
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -70,17 +71,22 @@ struct WaveLds {
         float terms[kTermCap];    // per-chunk cost terms (candidate-major)
     };
     static_assert(kTermCap * 4 <= kKeyCap * 8, "terms share the key buffer");
-    double dist[kMaxK];
+    // the distances serve the weights only, which are done before the
+    // evaluation's active candidates are written
+    union {
+        double dist[kMaxK];
+        struct {                      // active candidates of the pruned evaluation
+            uint32_t act_xy[kWave];
+            uint32_t act_ms[kWave];   // map | 1 << 31 for mirrored (random) patterns
+            uint32_t act_idx[kWave];  // index in the reference's candidate list
+            float act_sum[kWave];
+        };
+    };
     int32_t nd[kMaxK];
     float w[kMaxK];
     uint32_t pat[kMaxK];
     uint32_t gpat[kMaxK];
     uint4 coh[kMaxK];
-    // active candidates of the pruned evaluation
-    uint32_t act_xy[kWave];
-    uint32_t act_ms[kWave];   // map | 1 << 31 for mirrored (random) patterns
-    uint32_t act_idx[kWave];  // index in the reference's candidate list
-    float act_sum[kWave];
     double avg;
     // team exchange: neighbour count (leader), best (cost, index) per member,
     // and (leader's copy) the team's least complete (cost, index) so far,

@@ -2314,7 +2314,10 @@ void tsg_tuning_default(tsg_tuning* t) {
     // (2048^2), 3% faster at 256^2 and 512^2: it saves the commit dispatch
     // (5.4 us) but each resolve dispatch grows by 7.6 us
     t->defer = 0;
-    t->persist = 1;
+    // 0: with three 8-wave workgroups resident per CU (unguided), the
+    // per-8-items launch measured 24.05 ms against 25.1 ms for the persistent
+    // one, which its 128 VGPRs hold at two workgroups
+    t->persist = 0;
     // 25.6 ms against 26.1 ms (1024-item heads of 256..2048 measured alike;
     // 4-wave heads 25.9, 2-wave heads 26.7)
     t->flow_head = 1024;

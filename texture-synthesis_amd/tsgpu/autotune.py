@@ -28,7 +28,7 @@ SPACE: Sequence[Tuple[str, Sequence[int]]] = (
     ("prefetch_max", (1024, 2048, 4096)),
     ("ahead", (1, 2)),
     ("defer", (1, 0)),
-    ("persist", (1, 0)),
+    ("persist", (0, 1)),
     ("flow_head", (1024, 0, 4096)),
 )
 
