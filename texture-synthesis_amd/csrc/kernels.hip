@@ -29,10 +29,16 @@ constexpr int kWavesPerWG = 8;
 constexpr int kSmallWaves = 16;  // items per batch of k_resolve_small
 constexpr int kKeyCap = 256;
 // Interleaved copies of each 256-entry cost table in LDS (lane l reads copy
-// l % kLutCopies). With 8 copies an unguided 8-wave workgroup needs 51 KB, so
-// three are resident per CU (24 waves); a guided one (two tables) 59 KB, two
-// per CU. 32 copies fitted a single guided workgroup per CU.
-constexpr int kLutCopies = 8;
+// l % copies): 8 for an unguided job, 4 of each of a guided job's two tables.
+// Either way an 8-wave workgroup needs 51 KB, so three are resident per CU
+// (24 waves). 32 copies fitted a single guided workgroup per CU.
+template <bool GUIDED>
+constexpr int lut_copies() {
+    return GUIDED ? 4 : 8;
+}
+// LDS of the cost tables: a guided job's two tables (4 copies each) take what
+// the unguided table (8 copies) takes.
+__host__ __device__ constexpr size_t lut_bytes(bool) { return 256 * 8 * sizeof(float); }
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -560,16 +566,17 @@ __device__ __forceinline__ bool coord_ok(const MapDesc& md, int x, int y) {
     return md.smask == nullptr || md.smask[(uint32_t)y * md.w + (uint32_t)x] != 0;
 }
 
+template <int C>
 __device__ __forceinline__ float lut4(const float* lut, uint32_t q, uint32_t p, int slot) {
     // four channel terms in R, G, B, A order into one accumulator
     int d0 = abs((int)(q & 255u) - (int)(p & 255u));
     int d1 = abs((int)((q >> 8) & 255u) - (int)((p >> 8) & 255u));
     int d2 = abs((int)((q >> 16) & 255u) - (int)((p >> 16) & 255u));
     int d3 = abs((int)(q >> 24) - (int)(p >> 24));
-    float s = lut[d0 * kLutCopies + slot];
-    s = s + lut[d1 * kLutCopies + slot];
-    s = s + lut[d2 * kLutCopies + slot];
-    s = s + lut[d3 * kLutCopies + slot];
+    float s = lut[d0 * C + slot];
+    s = s + lut[d1 * C + slot];
+    s = s + lut[d2 * C + slot];
+    s = s + lut[d3 * C + slot];
     return s;
 }
 
@@ -616,13 +623,14 @@ __device__ __forceinline__ void gather_px(const MapDesc* maps, const WaveLds* L,
 template <bool GUIDED>
 __device__ __forceinline__ float term_of(const WaveLds* L, const float* lut, const float* glut, int slot, uint32_t j,
                                          uint32_t p, uint32_t gp) {
-    float s = lut4(lut, L->pat[j], p, slot);
+    constexpr int C = lut_copies<GUIDED>();
+    float s = lut4<C>(lut, L->pat[j], p, slot);
     if (GUIDED) {
         const uint32_t q = L->gpat[j];
-        s = s + glut[abs((int)(q & 255u) - (int)(gp & 255u)) * kLutCopies + slot];
-        s = s + glut[abs((int)((q >> 8) & 255u) - (int)((gp >> 8) & 255u)) * kLutCopies + slot];
-        s = s + glut[abs((int)((q >> 16) & 255u) - (int)((gp >> 16) & 255u)) * kLutCopies + slot];
-        s = s + glut[abs((int)(q >> 24) - (int)(gp >> 24)) * kLutCopies + slot];
+        s = s + glut[abs((int)(q & 255u) - (int)(gp & 255u)) * lut_copies<GUIDED>() + slot];
+        s = s + glut[abs((int)((q >> 8) & 255u) - (int)((gp >> 8) & 255u)) * lut_copies<GUIDED>() + slot];
+        s = s + glut[abs((int)((q >> 16) & 255u) - (int)((gp >> 16) & 255u)) * lut_copies<GUIDED>() + slot];
+        s = s + glut[abs((int)(q >> 24) - (int)(gp >> 24)) * lut_copies<GUIDED>() + slot];
     }
     return __fmul_rn(s, L->w[j]);
 }
@@ -978,15 +986,15 @@ __device__ __forceinline__ void item_exact(const BatchArgs& a, const MapDesc* ma
                 const bool in = (uint32_t)X2 < md.w && (uint32_t)Y2 < md.h;
                 const uint32_t idx = (uint32_t)Y2 * md.w + (uint32_t)X2;
                 const uint32_t p = in ? md.px[idx] : kOutsidePx;
-                float s = lut4(lut, L->pat[j], p, slot);
+                float s = lut4<lut_copies<GUIDED>()>(lut, L->pat[j], p, slot);
                 if (GUIDED) {
                     const bool gin = (uint32_t)X2 < md.gw && (uint32_t)Y2 < md.gh;
                     const uint32_t gp = gin ? md.gpx[(uint32_t)Y2 * md.gw + (uint32_t)X2] : kOutsidePx;
                     const uint32_t q = L->gpat[j];
-                    s = s + glut[abs((int)(q & 255u) - (int)(gp & 255u)) * kLutCopies + slot];
-                    s = s + glut[abs((int)((q >> 8) & 255u) - (int)((gp >> 8) & 255u)) * kLutCopies + slot];
-                    s = s + glut[abs((int)((q >> 16) & 255u) - (int)((gp >> 16) & 255u)) * kLutCopies + slot];
-                    s = s + glut[abs((int)(q >> 24) - (int)(gp >> 24)) * kLutCopies + slot];
+                    s = s + glut[abs((int)(q & 255u) - (int)(gp & 255u)) * lut_copies<GUIDED>() + slot];
+                    s = s + glut[abs((int)((q >> 8) & 255u) - (int)((gp >> 8) & 255u)) * lut_copies<GUIDED>() + slot];
+                    s = s + glut[abs((int)((q >> 16) & 255u) - (int)((gp >> 16) & 255u)) * lut_copies<GUIDED>() + slot];
+                    s = s + glut[abs((int)(q >> 24) - (int)(gp >> 24)) * lut_copies<GUIDED>() + slot];
                 }
                 score = __fadd_rn(score, __fmul_rn(s, L->w[j]));
                 if (score == score) lastgood = score;
@@ -1106,7 +1114,7 @@ __device__ uint4 resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t 
                               int lane, const FlowArgs* flow = nullptr) {
     uint4 res = make_uint4(0, 0, 0, 0);
     WaveLds* L = Lt + member;
-    const int slot = lane % kLutCopies;
+    const int slot = lane % lut_copies<GUIDED>();
     const uint32_t rel = item - item0;
     const bool lead = member == 0;
     uint32_t flat = 0;
@@ -1312,10 +1320,11 @@ template <bool GUIDED>
 __device__ __forceinline__ WaveLds* setup_lds(const BatchArgs& a, unsigned char* smem, float*& lut, float*& glut,
                                               MapDesc* smaps, const MapDesc*& maps) {
     lut = reinterpret_cast<float*>(smem);
-    glut = lut + 256 * kLutCopies;
-    for (int t = threadIdx.x; t < 256 * kLutCopies; t += blockDim.x) {
-        lut[t] = a.lut[t / kLutCopies];
-        if (GUIDED) glut[t] = a.lut[256 + t / kLutCopies];
+    constexpr int C = lut_copies<GUIDED>();
+    glut = lut + 256 * C;
+    for (int t = threadIdx.x; t < 256 * C; t += blockDim.x) {
+        lut[t] = a.lut[t / C];
+        if (GUIDED) glut[t] = a.lut[256 + t / C];
     }
     maps = a.maps;
     if (a.maps && a.n_maps <= kMapCache) {
@@ -1325,7 +1334,7 @@ __device__ __forceinline__ WaveLds* setup_lds(const BatchArgs& a, unsigned char*
     int32_t* oc = offtab_lds();
     for (uint32_t t = threadIdx.x; t < kOffCache && t < a.offtab_n; t += blockDim.x) oc[t] = a.offtab[t];
     __syncthreads();
-    return reinterpret_cast<WaveLds*>(smem + (GUIDED ? 2 : 1) * 256 * kLutCopies * sizeof(float));
+    return reinterpret_cast<WaveLds*>(smem + lut_bytes(GUIDED));
 }
 
 // Profiling: the launch's execution span on the device clock, as the
@@ -1998,11 +2007,11 @@ hipError_t launch_stats(const StatsArgs& a, hipStream_t s) {
 }
 
 size_t resolve_lds_bytes(bool guided) {
-    return (guided ? 2 : 1) * 256 * kLutCopies * sizeof(float) + kWavesPerWG * sizeof(WaveLds);
+    return lut_bytes(guided) + kWavesPerWG * sizeof(WaveLds);
 }
 
 size_t resolve_small_lds_bytes(bool guided) {
-    return (guided ? 2 : 1) * 256 * kLutCopies * sizeof(float) + kSmallWaves * sizeof(WaveLds);
+    return lut_bytes(guided) + kSmallWaves * sizeof(WaveLds);
 }
 
 uint32_t small_batch_max() { return kSmallWaves; }
@@ -2080,7 +2089,7 @@ hipError_t launch_resolve_small(const BatchArgs& a, const uint32_t* bounds, cons
 }
 
 size_t flow_lds_bytes(uint32_t team, bool guided) {
-    return (guided ? 2 : 1) * 256 * kLutCopies * sizeof(float) + team * sizeof(WaveLds);
+    return lut_bytes(guided) + team * sizeof(WaveLds);
 }
 
 uint32_t persist_max_wgs(bool guided) {
