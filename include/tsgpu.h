@@ -73,7 +73,9 @@ typedef struct tsg_schedule {
 /* Launch structure of the device path. Results never depend on it (every
  * choice resolves the same batches against the same frozen states); it
  * trades launch count, latency and occupancy, and is what the autotuner
- * picks.
+ * picks. tsg_tuning_default gives: small_max 4, fuse_max 0, team_max 8,
+ * team_waves 4096, prefetch_max 1024, ahead 1, coop 0/0, flow 1, flow_wgs 0,
+ * flow_team 4, defer 0, persist 0, flow_head 1024, flow_head_team 8.
  *   small_max    batches of at most this many items run in the one-workgroup
  *                loop kernel (0..16; 0 = never)
  *   fuse_max     larger batches of at most this many items commit in their
@@ -113,7 +115,8 @@ typedef struct tsg_schedule {
  *   persist      1: a batch of single-wave items runs on resident
  *                workgroups, each over an equal slice whose items its waves
  *                take in turn (no wave waits for the slowest item of its
- *                workgroup); 0: one workgroup per 8 items
+ *                workgroup); 0: one workgroup per 8 items (three resident
+ *                per CU, where the persistent kernel's registers allow two)
  *   flow_head    leading items of each dependency-ordered run that go to a
  *                launch of their own, with teams of flow_head_team waves
  *                (the run's first items form its longest serial chain)
