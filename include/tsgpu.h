@@ -281,6 +281,17 @@ int tsg_batch_stats(tsg_batch* b, uint64_t* rounds, uint64_t* launches, uint64_t
 /* Destroys the batch and every job it still holds. */
 void tsg_batch_destroy(tsg_batch* b);
 
+/* Resampling filters of image 0.23's imageops::resize used on the path:
+ * Triangle (inpaint masks, ms.rs:242-263), CatmullRom (load_image,
+ * utils.rs:55-80), Gaussian (ImagePyramid levels, img_pyramid.rs:27-32). */
+enum tsg_filter { TSG_FILTER_TRIANGLE = 0, TSG_FILTER_CATMULLROM = 1, TSG_FILTER_GAUSSIAN = 2 };
+
+/* load_image's resize step (utils.rs:55-80: image::imageops::resize with
+ * the given filter, vertical pass to f32, horizontal pass clamped and
+ * rounded to u8) on the device: out is nw*nh*4 RGBA8. An image already of
+ * the asked size is copied unchanged, as load_image does (utils.rs:66). */
+int tsg_resize(tsg_ctx* ctx, const tsg_image* img, uint32_t nw, uint32_t nh, int filter, uint8_t* out);
+
 /* ---- test and tooling entry points (no counterpart in the reference API;
  *      they expose single steps of the path for parity tests) ---- */
 

@@ -9,6 +9,19 @@ for p in (ROOT, os.path.join(ROOT, "texture-synthesis_amd")):
         sys.path.insert(0, p)
 
 
+def pytest_addoption(parser):
+    parser.addoption("--runslow", action="store_true", help="also run the long CPU oracle tests marked slow")
+
+
+def pytest_collection_modifyitems(config, items):
+    if config.getoption("--runslow"):
+        return
+    skip = pytest.mark.skip(reason="long CPU oracle run: pass --runslow")
+    for it in items:
+        if "slow" in it.keywords:
+            it.add_marker(skip)
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (run on the GPU box)")
     config.addinivalue_line("markers", "slow: long CPU oracle runs")

@@ -2569,6 +2569,31 @@ int tsg_pyramid_build(tsg_ctx* ctx, const tsg_image* img, uint32_t levels, uint8
     return TSG_OK;
 }
 
+int tsg_resize(tsg_ctx* ctx, const tsg_image* img, uint32_t nw, uint32_t nh, int filter, uint8_t* out) {
+    if (!ctx || !valid_image(img) || !out || nw == 0 || nh == 0)
+        return fail(TSG_ERR_INVALID_ARGUMENT, "null argument or zero size");
+    if (filter < TSG_FILTER_TRIANGLE || filter > TSG_FILTER_GAUSSIAN)
+        return fail(TSG_ERR_INVALID_ARGUMENT, "unknown resampling filter");
+    const size_t n_in = (size_t)img->w * img->h, n_out = (size_t)nw * nh;
+    if (img->w == nw && img->h == nh) {  // load_image keeps an image of the asked size as it is (utils.rs:66)
+        std::memcpy(out, img->rgba, n_in * 4);
+        return TSG_OK;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const Filter f = filter == TSG_FILTER_TRIANGLE ? Filter{kern_triangle, 1.0f}
+                     : filter == TSG_FILTER_CATMULLROM ? Filter{kern_catmullrom, 2.0f}
+                                                       : Filter{kern_gaussian, 3.0f};
+    DBuf<uint32_t> src, dst;
+    HIP_TRY(src.alloc(n_in));
+    HIP_TRY(dst.alloc(n_out));
+    HIP_TRY(hipMemcpyAsync(src.p, img->rgba, n_in * 4, hipMemcpyHostToDevice, ctx->stream));
+    int rc = device_resize(src.p, img->w, img->h, dst.p, nw, nh, f, ctx->stream);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(out, dst.p, n_out * 4, hipMemcpyDeviceToHost));
+    return TSG_OK;
+}
+
 int tsg_knn_batch(tsg_ctx* ctx, const uint8_t* mask, uint32_t w, uint32_t h, uint32_t n_q, const uint32_t* q_flat,
                   uint32_t k, int32_t* nbrs, uint32_t* counts) {
     if (!ctx || !mask || !q_flat || !nbrs || !counts) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");

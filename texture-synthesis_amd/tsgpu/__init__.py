@@ -149,6 +149,7 @@ def lib():
     L.tsg_job_phase_clocks.argtypes = [vp, vp, u64]
     L.tsg_job_destroy.argtypes = [vp]
     L.tsg_pyramid_build.argtypes = [vp, P(Image), u32, vp]
+    L.tsg_resize.argtypes = [vp, P(Image), u32, u32, C.c_int, vp]
     L.tsg_knn_batch.argtypes = [vp, vp, u32, u32, u32, vp, u32, vp, vp]
     L.tsg_random_candidates.argtypes = [vp, u32, P(Image), P(Image), u64, u32, u32, u32, vp]
     L.tsg_score_batch.argtypes = [vp, u32, u32, vp, vp, vp, u32, P(Image), P(Image), P(Image), vp, vp, u32, u32,
@@ -698,26 +699,34 @@ class SessionBuilder:
         self._ctx = ctx
         return self
 
-    @staticmethod
-    def _sized(img: np.ndarray, dims: Optional[Dims]) -> np.ndarray:
+    def _sized(self, img: np.ndarray, dims: Optional[Dims]) -> np.ndarray:
+        """load_image(src, Some(dims)) (utils.rs:55-80): CatmullRom resize on
+        the device when the size differs, else the image unchanged."""
         if dims is None or (img.shape[1] == dims.width and img.shape[0] == dims.height):
             return img
-        raise TsgError(-11, "resizing inputs (load_image with CatmullRom) is the caller's step; "
-                            "pass images at their final size")
+        return resize(img, dims.width, dims.height, "catmullrom", self._ctx)
 
     def _out_dims(self) -> Dims:
         return self.inpaint[2] if self.inpaint is not None else self._output_size
 
+    def _in_dims(self) -> Optional[Dims]:
+        return self.inpaint[2] if self.inpaint is not None else self._resize_input
+
     def footprint_estimate(self) -> int:
         """Device bytes the built session holds while it runs under a Batch."""
         out = self._out_dims()
-        wh = np.array([[e.img.shape[1], e.img.shape[0]] for e in self.examples], np.uint32).reshape(-1)
+        ind = self._in_dims()
+
+        def wh_of(img):
+            return [ind.width, ind.height] if ind is not None else [img.shape[1], img.shape[0]]
+
+        wh = np.array([wh_of(e.img) for e in self.examples], np.uint32).reshape(-1)
         kinds = np.array([TSG_SAMPLE_IMAGE if not isinstance(e.sample_method, str) else
                           TSG_SAMPLE_IGNORE if e.sample_method == SampleMethod.Ignore else TSG_SAMPLE_ALL
                           for e in self.examples], np.int32)
         gwh = None
         if self.target_guide is not None and self.examples and all(e.guide is not None for e in self.examples):
-            gwh = np.array([[e.guide.shape[1], e.guide.shape[0]] for e in self.examples], np.uint32).reshape(-1)
+            gwh = np.array([wh_of(e.guide) for e in self.examples], np.uint32).reshape(-1)
         v = C.c_uint64()
         _check(lib().tsg_footprint_estimate(C.byref(self.params), len(self.examples), _ptr(wh), _ptr(kinds),
                                             _ptr(gwh), out.width, out.height, C.byref(v)))
@@ -767,6 +776,20 @@ def pyramid(img: np.ndarray, levels: int, ctx: Optional[Context] = None) -> np.n
     h, w = I.keep[0].shape[:2]
     out = np.zeros((max(levels, 1), h, w, 4), np.uint8)
     _check(lib().tsg_pyramid_build(ctx._h, I.arr, levels, _ptr(out)))
+    return out
+
+
+FILTERS = {"triangle": 0, "catmullrom": 1, "gaussian": 2}
+
+
+def resize(img: np.ndarray, width: int, height: int, filt: str = "catmullrom",
+           ctx: Optional[Context] = None) -> np.ndarray:
+    """image::imageops::resize as load_image applies it (utils.rs:55-80), on
+    the device: RGBA8 in, (height, width, 4) RGBA8 out."""
+    ctx = ctx or Context.default()
+    I = _Images([img])
+    out = np.zeros((height, width, 4), np.uint8)
+    _check(lib().tsg_resize(ctx._h, I.arr, width, height, FILTERS[filt], _ptr(out)))
     return out
 
 
