@@ -306,6 +306,18 @@ int tsg_pyramid_build(tsg_ctx* ctx, const tsg_image* img, uint32_t levels, uint8
 int tsg_knn_batch(tsg_ctx* ctx, const uint8_t* mask, uint32_t w, uint32_t h, uint32_t n_q, const uint32_t* q_flat,
                   uint32_t k, int32_t* nbrs, uint32_t* counts);
 
+/* tsg_knn_batch without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, results and errors as tsg_knn_batch; every
+ * device buffer (staged mask and queries, tile bits, resolved list, lists,
+ * results) lives in a fixed 32 MiB device array per GPU, so the call
+ * returns TSG_ERR_UNSUPPORTED when w*h + w*h/8 + n_q*(12*k + 24) + 8 KiB
+ * (each part rounded up to 256 bytes) exceeds 32 MiB. All calls on one GPU
+ * share that array: calls that overlap on one GPU race. */
+int tsg_knn_batch_noalloc(tsg_ctx* ctx, const uint8_t* mask, uint32_t w, uint32_t h, uint32_t n_q,
+                          const uint32_t* q_flat, uint32_t k, int32_t* nbrs, uint32_t* counts);
+
 /* Random candidates (ms.rs:549-599) of stage items [item0, item0+n):
  * stream Pcg32::seed_from_u64(stage_seed + i + 1). maps / sample masks are
  * the filtered example maps. out: n*m (x, y, map) u32 triples. */

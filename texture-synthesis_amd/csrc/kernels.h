@@ -266,6 +266,13 @@ hipError_t launch_resolve(const BatchArgs& a, bool guided, hipStream_t s);
 uint32_t flow_max_wgs(uint32_t team, bool guided);
 hipError_t launch_resolve_flow(const BatchArgs& a, const FlowArgs& f, uint32_t wgs, uint32_t team, bool guided,
                                hipStream_t s);
+// Tile bitmask (one u64 per 8x8 tile) of a one-byte-per-pixel mask.
+hipError_t launch_mask_tiles(const uint8_t* mask, uint32_t W, uint32_t H, uint32_t tiles_x, uint32_t tiles_y,
+                             uint64_t* tiles, hipStream_t s);
+// Packed kNN offsets (dx & 0xffff | dy << 16) to (x, y) int32 pairs, zero
+// past each query's count.
+hipError_t launch_knn_unpack(const uint32_t* q_flat, const int32_t* packed, const uint32_t* counts, uint32_t n_q,
+                             uint32_t k, uint32_t W, int32_t* pairs, hipStream_t s);
 hipError_t launch_stats(const StatsArgs& a, hipStream_t s);
 hipError_t launch_rand(const RandArgs& a, hipStream_t s);
 hipError_t launch_commit(const CommitArgs& a, hipStream_t s);

@@ -1745,6 +1745,34 @@ __global__ void k_mask_or(const uint32_t* rlist, uint32_t lo, uint32_t hi, uint3
     if (tile_min) atomicMin(&tile_min[t], q);
 }
 
+// Resolved bits of a one-byte-per-pixel mask (nonzero = resolved), one u64
+// per 8x8 tile (bit = row * 8 + col): one thread per tile.
+__global__ void k_mask_tiles(const uint8_t* mask, uint32_t W, uint32_t H, uint32_t tiles_x, uint32_t tiles_y,
+                             uint64_t* tiles) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tiles_x * tiles_y) return;
+    const uint32_t x0 = (t % tiles_x) * 8u, y0 = (t / tiles_x) * 8u;
+    uint64_t bits = 0;
+    for (uint32_t r = 0; r < 8u && y0 + r < H; ++r)
+        for (uint32_t c = 0; c < 8u && x0 + c < W; ++c)
+            if (mask[(size_t)(y0 + r) * W + x0 + c]) bits |= 1ull << ((r << 3) | c);
+    tiles[t] = bits;
+}
+
+// kNN lists packed as (dx & 0xffff | dy << 16) per entry, to (x, y) int32
+// pairs of the query's neighbours; entries past a query's count are (0, 0).
+__global__ void k_knn_unpack(const uint32_t* q_flat, const int32_t* packed, const uint32_t* counts, uint32_t n_q,
+                             uint32_t k, uint32_t W, int32_t* pairs) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_q * k) return;
+    const uint32_t q = e / k, j = e % k;
+    const int32_t x = (int32_t)(q_flat[q] % W), y = (int32_t)(q_flat[q] / W);
+    const int32_t o = packed[e];
+    const bool v = j < counts[q];
+    pairs[2 * (size_t)e] = v ? x + (int32_t)(int16_t)(o & 0xffff) : 0;
+    pairs[2 * (size_t)e + 1] = v ? y + (o >> 16) : 0;
+}
+
 // Random candidates of items [item0, item1): Pcg32::seed_from_u64(stage
 // seed + i + 1), m draws of (map, then x and y until the sample mask accepts).
 __device__ __forceinline__ void rand_item(const RandArgs& a, uint32_t item) {
@@ -1995,6 +2023,23 @@ hipError_t launch_mask_or(const uint32_t* rlist, uint32_t lo, uint32_t hi, uint3
     if (hi <= lo) return hipSuccess;
     hipLaunchKernelGGL(k_mask_or, dim3((hi - lo + 255) / 256), dim3(256), 0, s, rlist, lo, hi, W, tiles_x, mask,
                        tile_min);
+    return hipGetLastError();
+}
+
+hipError_t launch_mask_tiles(const uint8_t* mask, uint32_t W, uint32_t H, uint32_t tiles_x, uint32_t tiles_y,
+                             uint64_t* tiles, hipStream_t s) {
+    const uint32_t n = tiles_x * tiles_y;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mask_tiles, dim3((n + 255) / 256), dim3(256), 0, s, mask, W, H, tiles_x, tiles_y, tiles);
+    return hipGetLastError();
+}
+
+hipError_t launch_knn_unpack(const uint32_t* q_flat, const int32_t* packed, const uint32_t* counts, uint32_t n_q,
+                             uint32_t k, uint32_t W, int32_t* pairs, hipStream_t s) {
+    const uint32_t n = n_q * k;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_knn_unpack, dim3((n + 255) / 256), dim3(256), 0, s, q_flat, packed, counts, n_q, k, W,
+                       pairs);
     return hipGetLastError();
 }
 

@@ -28,6 +28,16 @@
 
 using namespace tsg;
 
+// Device scratch of the entry points that allocate nothing
+// (tsg_knn_batch_noalloc): staging of inputs and outputs and every
+// intermediate buffer, carved per call. One copy per device, held until the
+// process ends. Host code never uses this name as a device address: each
+// context resolves the address on its device once (hipGetSymbolAddress in
+// tsg_ctx_create) and passes that pointer to copies and kernels. All calls on
+// one device share the array, so calls that overlap on one device race.
+constexpr size_t kScratchBytes = 32ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
 
 thread_local std::string g_err;
@@ -542,6 +552,7 @@ struct tsg_ctx {
     DBuf<int32_t> offtab;
     uint32_t offtab_n = 0;
     DBuf<float> dummy_lut;
+    unsigned char* scratch = nullptr;  // this device's copy of the file-scope scratch array
     std::mutex mu;
     // launch structures stored by the autotuner, by job shape (see tuning_key)
     std::map<std::vector<uint64_t>, tsg_tuning> tunings;
@@ -2077,6 +2088,9 @@ int tsg_ctx_create(int device, tsg_ctx** out) {
     c->offtab_n = (uint32_t)tab.size();
     HIP_TRY(c->dummy_lut.alloc(512));
     HIP_TRY(hipMemset(c->dummy_lut.p, 0, 512 * 4));
+    void* sp = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+    c->scratch = static_cast<unsigned char*>(sp);
     *out = c.release();
     return TSG_OK;
 }
@@ -2660,6 +2674,78 @@ int tsg_knn_batch(tsg_ctx* ctx, const uint8_t* mask, uint32_t w, uint32_t h, uin
             nbrs[((size_t)q * k + jn) * 2 + 1] = v ? y + (o >> 16) : 0;
         }
     }
+    return TSG_OK;
+}
+
+int tsg_knn_batch_noalloc(tsg_ctx* ctx, const uint8_t* mask, uint32_t w, uint32_t h, uint32_t n_q,
+                          const uint32_t* q_flat, uint32_t k, int32_t* nbrs, uint32_t* counts) {
+    if (!ctx || !mask || !q_flat || !nbrs || !counts) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    if (k == 0 || k > (uint32_t)kMaxK) return fail(TSG_ERR_UNSUPPORTED, "k must be in 1..64");
+    if (w == 0 || h == 0 || w > 16384 || h > 16384) return fail(TSG_ERR_INVALID_ARGUMENT, "bad size");
+    for (uint32_t q = 0; q < n_q; ++q)
+        if (q_flat[q] >= w * h) return fail(TSG_ERR_INVALID_ARGUMENT, "query outside the image");
+    if (n_q == 0) return TSG_OK;
+    // scratch layout: mask bytes | tile bits | resolved list | queries | counts | packed lists | results | pairs
+    const uint32_t tx = (w + 7) / 8, ty = (h + 7) / 8;
+    auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t o_mask = 0;
+    const size_t o_tiles = up(o_mask + (size_t)w * h);
+    const size_t o_rl = up(o_tiles + (size_t)tx * ty * 8);
+    const size_t o_q = up(o_rl + (size_t)kBruteMax * 4);
+    const size_t o_cnt = up(o_q + (size_t)n_q * 4);
+    const size_t o_packed = up(o_cnt + (size_t)n_q * 4);
+    const size_t o_res = up(o_packed + (size_t)n_q * k * 4);
+    const size_t o_pairs = up(o_res + (size_t)n_q * sizeof(uint4));
+    const size_t total = o_pairs + (size_t)n_q * k * 8;
+    if (total > kScratchBytes)
+        return fail(TSG_ERR_UNSUPPORTED, "mask and queries need " + std::to_string(total) +
+                                             " bytes of device scratch; the no-allocation entry point has " +
+                                             std::to_string(kScratchBytes) + " (use tsg_knn_batch)");
+    if (!ctx->scratch) return fail(TSG_ERR_DEVICE, "device scratch address unavailable");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    hipStream_t s = ctx->stream;
+    // the resolved list is read only while it holds at most kBruteMax points
+    uint32_t rl[kBruteMax];
+    uint32_t n_res = 0;
+    for (uint32_t i = 0; i < w * h; ++i)
+        if (mask[i]) {
+            if (n_res < kBruteMax) rl[n_res] = i;
+            ++n_res;
+        }
+    unsigned char* base = ctx->scratch;
+    HIP_TRY(hipMemcpyAsync(base + o_mask, mask, (size_t)w * h, hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_mask_tiles(base + o_mask, w, h, tx, ty, reinterpret_cast<uint64_t*>(base + o_tiles), s));
+    if (n_res > 0 && n_res <= kBruteMax)
+        HIP_TRY(hipMemcpyAsync(base + o_rl, rl, (size_t)n_res * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(base + o_q, q_flat, (size_t)n_q * 4, hipMemcpyHostToDevice, s));
+    BatchArgs a{};
+    a.W = w;
+    a.H = h;
+    a.tiles_x = tx;
+    a.tiles_y = ty;
+    a.mask = reinterpret_cast<const uint64_t*>(base + o_tiles);
+    a.work = reinterpret_cast<const uint32_t*>(base + o_q);
+    a.rlist = reinterpret_cast<const uint32_t*>(base + o_rl);
+    a.n_res = n_res;
+    a.offtab = ctx->offtab.p;
+    a.offtab_n = ctx->offtab_n;
+    a.lut = ctx->dummy_lut.p;
+    a.item0 = 0;
+    a.item1 = n_q;
+    a.k = k;
+    a.result = reinterpret_cast<uint4*>(base + o_res);
+    a.dbg_nbrs = reinterpret_cast<int32_t*>(base + o_packed);
+    a.dbg_nbr_count = reinterpret_cast<uint32_t*>(base + o_cnt);
+    a.knn_only = 1;
+    HIP_TRY(launch_resolve(a, false, s));
+    HIP_TRY(launch_knn_unpack(reinterpret_cast<const uint32_t*>(base + o_q),
+                              reinterpret_cast<const int32_t*>(base + o_packed),
+                              reinterpret_cast<const uint32_t*>(base + o_cnt), n_q, k, w,
+                              reinterpret_cast<int32_t*>(base + o_pairs), s));
+    HIP_TRY(hipMemcpyAsync(nbrs, base + o_pairs, (size_t)n_q * k * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(counts, base + o_cnt, (size_t)n_q * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
     return TSG_OK;
 }
 

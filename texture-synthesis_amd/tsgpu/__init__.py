@@ -151,6 +151,7 @@ def lib():
     L.tsg_pyramid_build.argtypes = [vp, P(Image), u32, vp]
     L.tsg_resize.argtypes = [vp, P(Image), u32, u32, C.c_int, vp]
     L.tsg_knn_batch.argtypes = [vp, vp, u32, u32, u32, vp, u32, vp, vp]
+    L.tsg_knn_batch_noalloc.argtypes = [vp, vp, u32, u32, u32, vp, u32, vp, vp]
     L.tsg_random_candidates.argtypes = [vp, u32, P(Image), P(Image), u64, u32, u32, u32, vp]
     L.tsg_score_batch.argtypes = [vp, u32, u32, vp, vp, vp, u32, P(Image), P(Image), P(Image), vp, vp, u32, u32,
                                   u32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, u32]
@@ -801,6 +802,18 @@ def knn(mask: np.ndarray, queries: np.ndarray, k: int, ctx: Optional[Context] = 
     nbrs = np.zeros((len(q), k, 2), np.int32)
     counts = np.zeros(len(q), np.uint32)
     _check(lib().tsg_knn_batch(ctx._h, _ptr(mask), w, h, len(q), _ptr(q), k, _ptr(nbrs), _ptr(counts)))
+    return nbrs, counts
+
+
+def knn_noalloc(mask: np.ndarray, queries: np.ndarray, k: int, ctx: Optional[Context] = None):
+    """knn through tsg_knn_batch_noalloc (no device allocation per call)."""
+    ctx = ctx or Context.default()
+    mask = np.ascontiguousarray(mask, np.uint8)
+    h, w = mask.shape
+    q = np.ascontiguousarray(queries, np.uint32)
+    nbrs = np.zeros((len(q), k, 2), np.int32)
+    counts = np.zeros(len(q), np.uint32)
+    _check(lib().tsg_knn_batch_noalloc(ctx._h, _ptr(mask), w, h, len(q), _ptr(q), k, _ptr(nbrs), _ptr(counts)))
     return nbrs, counts
 
 
