@@ -31,12 +31,19 @@ THREADS = min(16, os.cpu_count() or 1)
 
 
 def digest(g):
+    """(digest of the output maps and the resolved list, digest of the stage statistics)"""
     h = hashlib.blake2b(digest_size=16)
     for a in (g.image, g.coord, g.ids, g.resolved_flat, g.resolved_score):
         h.update(np.ascontiguousarray(a).tobytes())
+    hs = hashlib.blake2b(digest_size=16)
     for a in g.stage_stats:
-        h.update(np.ascontiguousarray(a).tobytes())
-    return h.hexdigest()
+        hs.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest(), hs.hexdigest()
+
+
+def same_digest(got, want, what):
+    assert got[0] == want[0], f"{what}: output maps differ from the run alone"
+    assert got[1] == want[1], f"{what}: stage statistics differ from the run alone"
 
 
 def same_as_oracle(g, job):
@@ -108,7 +115,7 @@ def test_batched_mix_256_equals_alone_and_oracle(ctx):
     k50 = next(j for j, s in enumerate(specs) if s[0] == 512 and s[1] == 1024 and s[2] == 50)
     for j in range(len(specs)):
         g = got[tickets[j]]
-        assert digest(g) == alone[j], f"job {j} {specs[j]} differs from its run alone"
+        same_digest(digest(g), alone[j], f"job {j} {specs[j]}")
         if j in (k20, k50):
             e, o, k, sd = specs[j]
             print(f"case: oracle for batched job {j} {specs[j]}", flush=True)
@@ -161,7 +168,7 @@ def test_serving_mix_to_2048_equals_alone_and_oracle(ctx):
             got[f.rid] = f.result
     assert sorted(got) == list(range(len(reqs)))
     for i, r in enumerate(reqs):
-        assert digest(got[i]) == alone[i], f"request {i} {r} differs from its run alone"
+        same_digest(digest(got[i]), alone[i], f"request {i} {r}")
     srv.close()
     seed, side, _ = reqs[-1]
     print("case: oracle for the 2048^2 inpainting request", flush=True)
