@@ -16,13 +16,16 @@ the host's cores.
 Two more legs report in the same line:
 * "batched": 256 independent syntheses (examples uniform in {64..512}^2,
   outputs uniform in {128..1024}^2, k in {20, 50}, m = 50, seeds 4000+j)
-  run together through the multi-job engine, against the same 256 run one
-  after another through Session.run; aggregate output Mpix/s of both, and
-  whether every batched result equals its sequential twin bit for bit.
+  run together through the work-queue engine, against the same 256 run one
+  after another; aggregate output Mpix/s of both with the outputs resident
+  (as the main line), the download-inclusive rate apart, and whether every
+  batched result equals its sequential twin bit for bit.
 * "serving": a Poisson request loop at a fixed rate; outputs log-uniform
   over 128^2..2048^2 from seeded 256^2 examples, 30% inpainting with 1-3
-  elliptical holes covering 20-40%; admission by device-memory footprint;
-  p50/p99 latency (arrival to result on the host) and output Mpix/s.
+  elliptical holes covering 20-40% (the example resized to the output with
+  CatmullRom); admission by device-memory footprint; p50/p99 latency
+  (arrival to result on the host) and output Mpix/s. A second, shorter run
+  under a tight budget shows admission held back by memory.
 """
 import hashlib
 import argparse
@@ -53,16 +56,56 @@ def build_session(ts, ex, profile=0):
     return sess
 
 
-def cpu_baseline(ex, threads):
+def host_cpu():
+    """CPU model and the cores this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return model, avail
+
+
+def cpu_baseline(ex, threads, runs=3):
+    """The oracle on the host's cores: the full 1024^2 workload under the
+    same Jacobi schedule on `threads` threads (median of `runs`), and the
+    reference's own CPU case, 128^2 -> 256^2 at one thread (one pixel per
+    batch: the reference's max_thread_count(1) semantics, seed 211)."""
     import tsgpu as ts
     from oracle import refcpu as R
-    job = R.Job(R.default_params(seed=0), [ex], 1024, 1024)
-    t0 = time.perf_counter()
-    job.run(threads=threads, **ts.default_schedule())
-    dt = time.perf_counter() - t0
+    from tsgpu import synth
+    ts_all = []
+    for _ in range(runs):
+        job = R.Job(R.default_params(seed=0), [ex], 1024, 1024)
+        t0 = time.perf_counter()
+        job.run(threads=threads, **ts.default_schedule())
+        ts_all.append(time.perf_counter() - t0)
+    dt = statistics.median(ts_all)
+    c1 = synth.example(211, 128, 128)
+    t1 = []
+    for _ in range(runs):
+        job = R.Job(R.default_params(seed=211), [c1], 256, 256)
+        t0 = time.perf_counter()
+        job.run(threads=1, px_per_batch=1, redo_shift=0, new_shift=0, max_batch=0, first_new_shift=0)
+        t1.append(time.perf_counter() - t0)
+    d1 = statistics.median(t1)
+    model, avail = host_cpu()
     return dict(value=round(1024 * 1024 / dt / 1e6, 4), unit="Mpix/s", cores=threads, kind="port",
-                sample=f"one full 1024^2 synthesis of the same workload on {threads} host threads "
-                       f"(oracle, same Jacobi schedule), {dt:.1f} s")
+                cpu_model=model, cores_available=avail,
+                sample=f"full 1024^2 syntheses of the same workload on {threads} host threads (oracle, same "
+                       f"Jacobi schedule), median of {runs}: " + ", ".join(f"{t:.2f}" for t in ts_all) + " s",
+                one_thread_c1={"value": round(256 * 256 / d1 / 1e6, 4), "unit": "Mpix/s", "cores": 1,
+                               "workload": "128^2 sinusoid+noise example (seed 211) -> 256^2, one pixel per batch "
+                                           "(max_thread_count(1) semantics), seed 211",
+                               "seconds": [round(t, 2) for t in t1]})
 
 
 def digest(g):
@@ -91,28 +134,30 @@ def batched_leg(ts, synth, n):
                 .output_size(ts.Dims.square(o)) for x, (e, o, k, sd) in zip(exs, specs)]
 
     px = sum(o * o for _, o, _, _ in specs)
-    # timed: the syntheses and their downloads; the result digests after
+    # timed, as the plain line: the syntheses with their outputs left in
+    # device memory (the downloads are timed apart); the digests after
     sess = [b.build() for b in builders()]
     t0 = time.perf_counter()
-    res = [s.run() for s in sess]
+    for s in sess:
+        s.run_resident()
     t_seq = time.perf_counter() - t0
-    seq = [digest(g) for g in res]
-    del sess, res
+    seq = []
+    for s in sess:
+        seq.append(digest(s.download()))
+        s.close()
+    del sess
     sess = [b.build() for b in builders()]
     batch = ts.Batch(ts.Context.default(), max_active=n)
     t0 = time.perf_counter()
     tickets = [batch.submit(s) for s in sess]
-    # results are taken (downloaded) as jobs finish, while later rounds run
-    done = {}
-    while len(done) < n:
-        batch.step(8)
-        for t, status in batch.poll():
-            done[t] = batch.take(t)
-    res = [done[t] for t in tickets]
+    batch.run_all()
     t_bat = time.perf_counter() - t0
-    got = [digest(g) for g in res]
-    del res
+    done = dict(batch.poll())
+    assert sorted(done) == sorted(tickets) and not any(done.values())
+    got = [digest(batch.take(t)) for t in tickets]
+    t_pcie = time.perf_counter() - t0
     st = batch.stats()
+    prof = batch.engine_profile()
     batch.close()
     return {
         "workload": f"{n} syntheses: examples uniform in {{64,128,256,512}}^2, outputs uniform in "
@@ -122,8 +167,11 @@ def batched_leg(ts, synth, n):
         "sequential_value": round(px / t_seq / 1e6, 3),
         "speedup_vs_sequential": round(t_seq / t_bat, 2),
         "seconds": round(t_bat, 3), "sequential_seconds": round(t_seq, 3),
+        "pcie_inclusive": {"value": round(px / t_pcie / 1e6, 3), "unit": "Mpix/s", "seconds": round(t_pcie, 3),
+                           "note": "the batched syntheses with every result downloaded to host memory"},
         "identical_to_alone": seq == got,
-        "rounds": st["rounds"], "launches": st["launches"],
+        "launches": st["launches"], "engine_rounds": st["rounds"],
+        "engine_workgroup_ms": {k: (round(v, 1) if isinstance(v, float) else v) for k, v in prof.items()},
     }
 
 
@@ -155,30 +203,31 @@ def guided_leg(ts, synth, steps):
     }
 
 
-def serving_requests(ts, synth, n, seed=5000):
+def serving_request(ts, synth, sd, side, inpaint):
+    """A request on a seeded 256^2 example; an inpainting one is resized to
+    the output size by the builder (load_image's CatmullRom, utils.rs:55-80)."""
+    ex = synth.example(sd, 256, 256)
+    b = ts.Session.builder().seed(sd)
+    if inpaint:
+        m = synth.ellipse_mask(sd, side, side)
+        b.inpaint_example(m, ts.Example(ex).set_sample_method(m), ts.Dims.square(side))
+    else:
+        b.add_example(ex).output_size(ts.Dims.square(side))
+    return b
+
+
+def serving_specs(n, seed=5000):
     import numpy as np
     rng = np.random.default_rng(seed)
     sides = np.exp(rng.uniform(np.log(128), np.log(2048), n)).astype(int)
     inpaint = rng.random(n) < 0.3
-    reqs = []
-    for i in range(n):
-        side, sd = int(sides[i]), seed + i
-        ex = synth.example(sd, 256, 256)
-        b = ts.Session.builder().seed(sd)
-        if inpaint[i]:
-            idx = (np.arange(side) * 256) // side
-            m = synth.ellipse_mask(sd, side, side)
-            b.inpaint_example(m, ts.Example(ex[idx][:, idx]).set_sample_method(m), ts.Dims.square(side))
-        else:
-            b.add_example(ex).output_size(ts.Dims.square(side))
-        reqs.append((b, side * side, bool(inpaint[i])))
-    return reqs
+    return [(seed + i, int(sides[i]), bool(inpaint[i])) for i in range(n)]
 
 
 def serving_leg(ts, synth, n, qps, budget_gib):
     from tsgpu import serve
     import numpy as np
-    reqs = serving_requests(ts, synth, n)
+    reqs = [(sp, sp[1] * sp[1], sp[2]) for sp in serving_specs(n)]
     gaps = np.random.default_rng(5999).exponential(1.0 / qps, n)
     arrivals = np.cumsum(gaps) - gaps[0]
     srv = serve.Server(ts.Context.default(), hbm_budget=int(budget_gib * (1 << 30)), max_active=256,
@@ -190,7 +239,7 @@ def serving_leg(ts, synth, n, qps, budget_gib):
     while i < n or not srv.idle():
         now = time.perf_counter() - t0
         while i < n and arrivals[i] <= now:
-            srv.submit(reqs[i][0], arrival=t0 + arrivals[i])
+            srv.submit(serving_request(ts, synth, *reqs[i][0]), arrival=t0 + arrivals[i])
             i += 1
         if srv.idle():
             time.sleep(max(0.0, min(0.01, arrivals[i] - now)))
@@ -213,7 +262,7 @@ def serving_leg(ts, synth, n, qps, budget_gib):
         "seconds": round(t_end - t0, 3), "errors": len(errors),
         "busy_fraction": round(busy / (t_end - t0), 3),
         "hbm_budget_gib": budget_gib, "peak_admitted_gib": round(srv.peak_in_use / (1 << 30), 3),
-        "memory_holds": srv.memory_holds,
+        "memory_holds": srv.memory_holds, "rejected": srv.rejected,
     }
 
 
@@ -252,13 +301,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="oracle threads (0: the cores this process may use, at most 16)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="time the library's default launch structure")
     ap.add_argument("--batched", type=int, default=256, help="syntheses in the batched leg (0 = skip)")
     ap.add_argument("--guided", type=int, default=3, help="runs of the guided leg (0 = skip)")
-    ap.add_argument("--serve-requests", type=int, default=480, help="requests in the serving leg (0 = skip)")
+    ap.add_argument("--serve-requests", type=int, default=2000, help="requests in the serving leg (0 = skip)")
+    ap.add_argument("--serve-tight-requests", type=int, default=300,
+                    help="requests of a second serving run under a tight memory budget (0 = skip)")
+    ap.add_argument("--serve-tight-gib", type=float, default=0.5)
     ap.add_argument("--serve-qps", type=float, default=18.0,
                     help="Poisson arrival rate; 18/s keeps the engine busy about 70%% of the time")
     ap.add_argument("--serve-budget-gib", type=float, default=4.0)
@@ -361,8 +414,12 @@ def main():
         out["batched"]["vs_plain"] = round(out["batched"]["value"] / value, 2)
     if args.serve_requests:
         out["serving"] = serving_leg(ts, synth, args.serve_requests, args.serve_qps, args.serve_budget_gib)
+    if args.serve_tight_requests:
+        out["serving_tight_budget"] = serving_leg(ts, synth, args.serve_tight_requests, args.serve_qps,
+                                                  args.serve_tight_gib)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(ex, args.cpu_threads)
+        threads = args.cpu_threads or min(16, host_cpu()[1])
+        out["cpu_baseline"] = cpu_baseline(ex, threads)
         out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
     else:
         out["cpu_baseline"] = None

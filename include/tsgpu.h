@@ -37,7 +37,9 @@ enum tsg_status {
     TSG_ERR_UNSUPPORTED = -11,           /* outside the device path's limits */
     TSG_ERR_DEVICE = -20,                /* HIP runtime failure */
     TSG_ERR_OUT_OF_MEMORY = -21,         /* device allocation failed */
-    TSG_ERR_SAMPLING = -22               /* a sample mask admits no pixel */
+    TSG_ERR_SAMPLING = -22,              /* a sample mask admits no pixel */
+    TSG_ERR_CANCELLED = -23,             /* the job was cancelled before it finished (tsg_batch_cancel) */
+    TSG_ERR_ADMISSION_REJECTED = -24     /* the job can never fit the caller's device-memory budget */
 };
 
 /* GeneratorParams (ms.rs:17-42) lowered from Parameters (lib.rs:327-375).
@@ -278,6 +280,11 @@ int tsg_batch_run_all(tsg_batch* b);
 int tsg_batch_poll(tsg_batch* b, uint64_t* tickets, int* status, uint32_t cap, uint32_t* n);
 int tsg_batch_take(tsg_batch* b, uint64_t ticket, tsg_job** job);
 int tsg_batch_stats(tsg_batch* b, uint64_t* rounds, uint64_t* launches, uint64_t* units);
+/* Cancels a job that has not finished: a waiting job never starts, a
+ * running one stops between launches. Either is then reported finished by
+ * tsg_batch_poll with status TSG_ERR_CANCELLED and taken back as usual (its
+ * results are not usable). The other jobs are not affected. */
+int tsg_batch_cancel(tsg_batch* b, uint64_t ticket);
 /* Destroys the batch and every job it still holds. */
 void tsg_batch_destroy(tsg_batch* b);
 
@@ -294,6 +301,30 @@ int tsg_resize(tsg_ctx* ctx, const tsg_image* img, uint32_t nw, uint32_t nh, int
 
 /* ---- test and tooling entry points (no counterpart in the reference API;
  *      they expose single steps of the path for parity tests) ---- */
+
+/* Where the engine's workgroup time went, summed over the batch's launches
+ * and workgroups: milliseconds searching for units (idle waits included),
+ * in resolve units, in other units, closing phases; then the counts of
+ * resolve units and other units. */
+int tsg_batch_engine_profile(tsg_batch* b, double* out6);
+/* Test hook of the work-queue engine: at most wgs_cap workgroups per launch
+ * (0: as many as are resident) and engine flags (bit 0: device-scope
+ * acquire and release around every work unit). */
+int tsg_batch_debug_engine(tsg_batch* b, uint32_t wgs_cap, uint32_t flags);
+
+/* Failure injection for tests of error handling: the library's next `count`
+ * device allocations fail as if the device were out of memory (0 turns it
+ * off). Process-wide. */
+int tsg_debug_fail_allocs(int64_t count);
+
+/* Device value of the neighbour weights' (float)exp(-1.0 * x) (ms.rs:1198-1203;
+ * the library's own double-double exp) for the sweep arguments x_i, i in
+ * [i0, i0 + n), n <= 2^28: t = ((i * 0x9E3779B97F4A7C15) mod 2^64 >> 11) *
+ * 2^-53, x_i = 8 t when i % 4 != 0, else 8 + 56 t (f64, no contraction).
+ * Test hook for comparing with the host C library's exp, as the reference
+ * calls it. tsg_debug_exp_sweep_host computes the same on the host. */
+int tsg_debug_exp_sweep(tsg_ctx* ctx, uint64_t i0, uint64_t n, float* out);
+int tsg_debug_exp_sweep_host(uint64_t i0, uint64_t n, float* out);
 
 /* ImagePyramid::new(img, Some(levels)) on the device: writes max(levels,1)
  * full-size RGBA8 levels, coarsest first (img_pyramid.rs:20-37). */

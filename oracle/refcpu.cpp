@@ -1022,6 +1022,41 @@ void orc_range_u64(uint64_t state, uint64_t stream, uint64_t lo, uint64_t hi, ui
     Pcg32 g = Pcg32::make(state, stream);
     for (uint32_t i = 0; i < n; ++i) out[i] = range_u64(g, lo, hi);
 }
+/* The neighbour weights' (float)exp(-1.0 * x) (ms.rs:1198-1203) with the host C
+ * library's exp for the sweep arguments x_i (same formula as the device
+ * test hook tsg_debug_exp_sweep), compared with `dev`: returns the number of
+ * differing results; the first index that differs goes to *first_bad. */
+uint64_t orc_exp_compare(uint64_t i0, uint64_t n, const float* dev, uint32_t threads, uint64_t* first_bad) {
+    if (threads == 0) threads = 1;
+    std::vector<uint64_t> bad(threads, 0), first(threads, UINT64_MAX);
+    std::vector<std::thread> th;
+    for (uint32_t t = 0; t < threads; ++t)
+        th.emplace_back([&, t] {
+            const uint64_t lo = n * t / threads, hi = n * (t + 1) / threads;
+            for (uint64_t r = lo; r < hi; ++r) {
+                const uint64_t i = i0 + r;
+                const double u = (double)((i * 0x9E3779B97F4A7C15ull) >> 11) * 0x1p-53;
+                const double x = (i & 3u) ? u * 8.0 : 8.0 + u * 56.0;
+                const float w = (float)std::exp(-1.0 * x);
+                uint32_t a, b;
+                memcpy(&a, &w, 4);
+                memcpy(&b, &dev[r], 4);
+                if (a != b) {
+                    if (first[t] == UINT64_MAX) first[t] = i;
+                    ++bad[t];
+                }
+            }
+        });
+    for (auto& x : th) x.join();
+    uint64_t total = 0, f = UINT64_MAX;
+    for (uint32_t t = 0; t < threads; ++t) {
+        total += bad[t];
+        f = std::min(f, first[t]);
+    }
+    if (first_bad) *first_bad = f;
+    return total;
+}
+
 uint64_t orc_pick_index(uint64_t seed, uint64_t len) {
     Pcg32 g = Pcg32::seed_from_u64(seed);
     return range_u64(g, 0, len);

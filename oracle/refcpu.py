@@ -84,6 +84,8 @@ def lib():
         L.orc_pcg_from_seed_u64.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
         L.orc_range_u32.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]
         L.orc_range_u64.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32, C.c_void_p]
+        L.orc_exp_compare.argtypes = [C.c_uint64, C.c_uint64, C.c_void_p, C.c_uint32, C.c_void_p]
+        L.orc_exp_compare.restype = C.c_uint64
         L.orc_pick_index.argtypes = [C.c_uint64, C.c_uint64]
         L.orc_pick_index.restype = C.c_uint64
         L.orc_luts_full.argtypes = [C.c_float, C.c_float, C.c_int, C.c_void_p, C.c_void_p]
@@ -133,6 +135,14 @@ def range_u64(state: int, stream: int, lo: int, hi: int, n: int) -> np.ndarray:
     out = np.zeros(n, np.uint64)
     lib().orc_range_u64(state, stream, lo, hi, n, _ptr(out))
     return out
+
+
+def exp_compare(i0: int, dev: np.ndarray, threads: int = 8):
+    """(mismatches, first differing index) of the device's weight exps against the host's."""
+    dev = np.ascontiguousarray(dev, np.float32)
+    first = C.c_uint64()
+    n = int(lib().orc_exp_compare(i0, len(dev), _ptr(dev), threads, C.byref(first)))
+    return n, first.value
 
 
 def pick_index(seed: int, length: int) -> int:

@@ -35,11 +35,13 @@ def same(a, b):
             and np.array_equal(a.resolved_score.view(np.uint32), b.resolved_score.view(np.uint32)))
 
 
-@pytest.mark.parametrize("budget_mb,max_active", [(4096, 64), (6, 3)])
+@pytest.mark.parametrize("budget_mb,max_active", [(4096, 64), (0, 3)])
 def test_serving_results_equal_alone(ctx, budget_mb, max_active):
     reqs = mix()
     alone = [request(*r).build().run() for r in reqs]
-    srv = serve.Server(ctx, hbm_budget=budget_mb << 20, max_active=max_active, rounds_per_pump=5)
+    # 0: a budget that just fits the largest request, so admissions wait for memory
+    budget = budget_mb << 20 if budget_mb else max(request(*r).footprint_estimate() for r in reqs)
+    srv = serve.Server(ctx, hbm_budget=budget, max_active=max_active, rounds_per_pump=5)
     got = {}
     pending = list(enumerate(reqs))
     # arrivals interleaved with engine rounds: two requests join per pump
@@ -57,7 +59,9 @@ def test_serving_results_equal_alone(ctx, budget_mb, max_active):
     assert sorted(got) == list(range(len(reqs)))
     for i, r in enumerate(reqs):
         assert same(got[i], alone[i]), f"request {i} {r}"
-    assert srv.in_use == 0
+    assert srv.in_use == 0 and srv.rejected == 0
+    if not budget_mb:
+        assert srv.memory_holds > 0
     srv.close()
 
 

@@ -224,6 +224,127 @@ struct StatsArgs {
 };
 
 hipError_t launch_knn_prefetch(const PrefetchArgs& a, hipStream_t s);
+
+// Stage statistics in a fixed order: chunks of kStatsChunk items, each summed
+// by 256 threads (thread t takes items t, t + 256, ... in turn, then a tree
+// over the threads), then the chunks' partial sums in chunk order. The
+// single-job path and the work-queue engine use the same arithmetic, so a
+// job's statistics do not depend on which ran it.
+constexpr uint32_t kStatsChunk = 4096;
+hipError_t launch_stats_chunks(const float* item_score, uint32_t n, double* part, unsigned long long* part_n,
+                               double* sum, unsigned long long* count, hipStream_t s);
+
+// ---- work-queue engine: many jobs in one launch of persistent workgroups ----
+// Each job brings a program: a list of phases run in order, each a parallel
+// loop over `units` work units (a phase starts once every unit of the one
+// before it has finished). Workgroups take units from the jobs' cursors; the
+// workgroup that finishes a phase's last unit does the phase's closing work
+// and opens the next phase. A job runs on one XCD, so its data stays in
+// that XCD's L2 from phase to phase.
+enum QKind : uint32_t {
+    Q_RESOLVE = 0,   // batch items [i0, i1), 8 / team items per unit
+    Q_COMMIT = 1,    // batch items [i0, i1), kQElems per unit
+    Q_RAND = 2,      // random candidates of items [i0, i1) into the job's chunk (rc_base = i0)
+    Q_RECOLOR = 3,   // resolved-list positions [i0, i1) recoloured from the stage level
+    Q_STATS = 4,     // stage items [i0, i1), kStatsChunk per unit; the closing workgroup sums the chunks
+    Q_CLEAR = 5,     // tiles [i0, i1) of the prefetch window masks
+    Q_MASKOR = 6,    // resolved-list positions [i0, i1) into the window masks
+    Q_PREFETCH = 7,  // neighbour lists of items [i0, i1), 8 per unit
+};
+constexpr uint32_t kQElems = 512;
+
+struct QPhase {
+    uint32_t kind;
+    uint32_t stage;    // slot of the job's stage arguments
+    uint32_t i0, i1;
+    uint32_t n_res;    // resolve: resolved pixels at the batch start
+    uint32_t team;     // resolve: waves per item
+    uint32_t rc_base;  // resolve: item of the candidate chunk's first entry
+    uint32_t pre_base, pre_end;  // resolve: items with neighbour lists (pre_end 0: none);
+                                 // prefetch: pre_base = offset of the window's n_at, pre_end = 1
+                                 // for a new-pixel window (mask_new, tile_min, rank)
+    uint32_t units;
+    uint32_t fuse;     // resolve: the closing workgroup commits the batch (no commit phase)
+    uint32_t arg;      // index of the phase's batch arguments in the launch's argument array
+};
+
+struct QJob {
+    const BatchArgs* stages;   // per stage slot
+    const QPhase* phases;      // this launch's program
+    uint32_t n_phases;
+    uint32_t xcc;
+    uint2* rc;
+    uint4* result;
+    uint32_t* overflow;
+    uint64_t* mend;
+    uint32_t* tmin;
+    const uint32_t* n_at;
+    int32_t* pre_nbrs;
+    uint32_t* pre_cnt;
+    const uint32_t* rank;
+    double* st_part;
+    unsigned long long* st_part_n;
+    double* st_sum;            // per stage slot
+    unsigned long long* st_cnt;
+};
+
+// Per job, per launch: cursor = phase << 32 | units handed out; finished =
+// units of the current phase done. Zero at the launch start. Only the job's
+// XCD writes them; each job's pair has a 256-byte block of its own, so no
+// cache line is written from two XCDs (their L2s are not coherent).
+struct QDyn {
+    unsigned long long cursor;
+    uint32_t finished;
+    uint32_t pad[61];
+};
+static_assert(sizeof(QDyn) == 256, "one 256-byte block per job");
+
+// Per XCD, in a 256-byte block of its own: jobs whose program is not through,
+// units finished (idle workgroups give up, setting `error`, after idle_ticks
+// of the 100 MHz clock without progress on their XCD).
+struct QXcd {
+    uint32_t active;
+    uint32_t progress;
+    uint32_t error;
+    uint32_t units_resolve;          // units done, by kind (resolve, other)
+    uint32_t units_other;
+    uint32_t pad0;
+    // 100 MHz clock ticks summed over the XCD's workgroups: searching for a
+    // unit (including idle waits), resolve units, other units, phase closing
+    unsigned long long t_find, t_resolve, t_other, t_close;
+    uint32_t pad[50];
+};
+static_assert(sizeof(QXcd) == 256, "one 256-byte block per XCD");
+
+// Per launch: jobs of XCD x are [job0[x], job0[x + 1]) (read only).
+struct QCtl {
+    uint32_t job0[9];
+    uint32_t pad[55];
+    QXcd x[8];
+};
+static_assert(sizeof(QCtl) == 256 * 9, "job table offsets, then one block per XCD");
+
+// The launch's batch arguments: every resolve phase has its own entry (its
+// batch, candidate chunk and neighbour-list window filled in), every other
+// phase shares its stage's entry.
+struct QEngineArgs {
+    const QJob* jobs;
+    QDyn* dyn;
+    QCtl* ctl;
+    const int32_t* offtab;
+    uint32_t offtab_n;
+    uint32_t flags;          // reserved (0)
+    uint64_t idle_ticks;
+};
+
+// Workgroups of the engine launch resident on the device at once.
+uint32_t engine_max_wgs(bool guided);
+hipError_t launch_engine(const QEngineArgs& e, const BatchArgs* args, uint32_t wgs, bool guided, hipStream_t s);
+// (float)exp(-x_i) for the sweep arguments x_i, i in [i0, i0 + n) (test hook).
+hipError_t launch_exp_sweep(uint64_t i0, uint64_t n, float* out, hipStream_t s);
+// XCD of each of n workgroups of a probe launch (test hook: the engine's
+// job placement relies on the hardware XCC id).
+hipError_t launch_xcc_probe(uint32_t* out, uint32_t n, hipStream_t s);
 // random candidates of several jobs: unit u's items start at item_start[u]
 // of the launch's `total` threads (item_start[0] = 0)
 hipError_t launch_rand_multi(const RandArgs* units, const uint32_t* item_start, uint32_t n_units, uint32_t total,

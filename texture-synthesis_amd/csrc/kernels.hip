@@ -18,6 +18,7 @@
 #include <hip/hip_ext.h>
 #include <math.h>
 
+#include "exp_dd.h"
 #include "kernels.h"
 
 namespace tsg {
@@ -882,7 +883,7 @@ __device__ __forceinline__ void item_geometry(const BatchArgs& a, int x, int y, 
         L->avg = sum / (double)(4u * kf);
     }
     wave_sync();
-    if (lane < (int)kf) L->w[lane] = (float)exp(-1.0 * (L->dist[lane] / L->avg));
+    if (lane < (int)kf) L->w[lane] = (float)exp_dd(-1.0 * (L->dist[lane] / L->avg));
     wave_sync();
 }
 
@@ -1625,12 +1626,7 @@ __global__ void k_commit_multi(const BatchArgs* units, const uint32_t* item_star
 
 // Neighbour lists of items [item0, item1) ahead of their batches, one
 // wavefront per item (the resolve kernels read them instead of searching).
-__global__ __launch_bounds__(kWave* kWavesPerWG) void k_knn_prefetch(PrefetchArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    WaveLds* L = reinterpret_cast<WaveLds*>(smem) + threadIdx.x / kWave;
-    const int lane = threadIdx.x % kWave;
-    const uint32_t item = a.item0 + blockIdx.x * kWavesPerWG + threadIdx.x / kWave;
-    if (item >= a.item1) return;
+__device__ __forceinline__ void prefetch_item(const PrefetchArgs& a, WaveLds* L, uint32_t item, int lane) {
     const uint32_t rel = item - a.item0;
     const uint32_t flat = a.work[item];
     uint32_t n_at, bs = 0;
@@ -1668,6 +1664,15 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_knn_prefetch(PrefetchArg
         const uint64_t dm = __ballot(d);
         if (lane == 0) a.out_dep[rel] = dm;
     }
+}
+
+__global__ __launch_bounds__(kWave* kWavesPerWG) void k_knn_prefetch(PrefetchArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    WaveLds* L = reinterpret_cast<WaveLds*>(smem) + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    const uint32_t item = a.item0 + blockIdx.x * kWavesPerWG + threadIdx.x / kWave;
+    if (item >= a.item1) return;
+    prefetch_item(a, L, item, lane);
 }
 
 __global__ void k_unpack(UnpackArgs a) {
@@ -1925,6 +1930,389 @@ __global__ void k_stats(StatsArgs a) {
     }
 }
 
+// Sum and count of the non-negative scores of stage items [lo, hi) (at most
+// kStatsChunk), in a fixed order: threads 0..255 take items t, t + 256, ...
+// in turn, then a tree over the threads. Every thread of the block calls it
+// (threads from 256 on only take the barriers); red / redn: 256 entries of
+// LDS each. The result is valid in thread 0.
+__device__ __forceinline__ void stats_chunk(const float* score, uint32_t lo, uint32_t hi, double* red,
+                                            unsigned long long* redn, double& sum, unsigned long long& cnt) {
+    const uint32_t t = threadIdx.x;
+    if (t < 256) {
+        double s = 0.0;
+        unsigned long long c = 0;
+        for (uint32_t i = lo + t; i < hi; i += 256) {
+            const float v = score[i];
+            if (v >= 0.0f) {
+                s += (double)v;
+                c += 1;
+            }
+        }
+        red[t] = s;
+        redn[t] = c;
+    }
+    __syncthreads();
+    for (uint32_t o = 128; o > 0; o >>= 1) {
+        if (t < o) {
+            red[t] += red[t + o];
+            redn[t] += redn[t + o];
+        }
+        __syncthreads();
+    }
+    sum = red[0];
+    cnt = redn[0];
+}
+
+__global__ __launch_bounds__(256) void k_stats_part(const float* score, uint32_t n, double* part,
+                                                    unsigned long long* part_n) {
+    __shared__ double red[256];
+    __shared__ unsigned long long redn[256];
+    const uint32_t lo = blockIdx.x * kStatsChunk;
+    const uint32_t hi = min(n, lo + kStatsChunk);
+    double s;
+    unsigned long long c;
+    stats_chunk(score, lo, hi, red, redn, s, c);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = s;
+        part_n[blockIdx.x] = c;
+    }
+}
+
+__device__ __forceinline__ void stats_final(const double* part, const unsigned long long* part_n, uint32_t n_parts,
+                                            double* sum, unsigned long long* count) {
+    double s = 0.0;
+    unsigned long long c = 0;
+    for (uint32_t u = 0; u < n_parts; ++u) {
+        s += part[u];
+        c += part_n[u];
+    }
+    *sum = s;
+    *count = c;
+}
+
+__global__ void k_stats_final(const double* part, const unsigned long long* part_n, uint32_t n_parts, double* sum,
+                              unsigned long long* count) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) stats_final(part, part_n, n_parts, sum, count);
+}
+
+// ---- work-queue engine ----
+
+// XCD (XCC) this wave runs on: hardware register XCC_ID, bits [3:0].
+__device__ __forceinline__ uint32_t xcc_id() {
+    return (uint32_t)__builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 7u;
+}
+
+// Acquire: following loads see what other workgroups of the XCD released
+// (device-scope acquire: the CU's L1 and the L2's non-coherent lines are
+// dropped).
+__device__ __forceinline__ void unit_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+
+// Release to the workgroups of the same XCD: the wave waits until its stores
+// have completed to the XCD's L2, which every CU of the XCD reads through.
+// A job's workgroups all run on its XCD, so the L2 write-back of a
+// device-scope release is not needed (it made the engine four times slower).
+__device__ __forceinline__ void unit_release() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// Weight-exp sweep (test hook): out[i - i0] = (float)exp_dd(-1.0 * x_i), the
+// expression of the neighbour weights in item_geometry, for the arguments
+// x_i = t * 8 when i % 4 != 0, else 8 + t * 56, with t = ((i * 0x9E3779B97F4A7C15)
+// >> 11) * 2^-53 in [0, 1): three quarters of the arguments in [0, 8), where
+// d / mean lies for real neighbour sets, the rest up to 64.
+__global__ void k_exp_sweep(uint64_t i0, uint64_t n, float* out) {
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const uint64_t i = i0 + r;
+    const double t = (double)((i * 0x9E3779B97F4A7C15ull) >> 11) * 0x1p-53;
+    const double x = (i & 3u) ? t * 8.0 : __dadd_rn(8.0, __dmul_rn(t, 56.0));
+    out[r] = (float)exp_dd(-1.0 * x);
+}
+
+__global__ void k_xcc_probe(uint32_t* out, uint32_t n) {
+    if (threadIdx.x == 0 && blockIdx.x < n) out[blockIdx.x] = xcc_id();
+}
+
+// Persistent workgroups (as many as are resident) that drain the programs of
+// the jobs placed on their XCD. Wave 0 finds a unit: its lanes read up to 64
+// jobs' cursors at a time, from the last job served on; a job whose current
+// phase still has units left is claimed with an atomic add on its cursor
+// (a claim that lands past the phase's units, because others took them in
+// between, is dropped and the search goes on). Then every wave works on the
+// unit, the workgroup counts it finished, and the workgroup that finishes a
+// phase's last unit closes it: a fused batch commit or the stage statistics'
+// final sum, then the next phase's cursor. A workgroup stops when every job of
+// its XCD is through its program, or after idle_ticks without any unit
+// finishing on its XCD (setting its XCD's error word).
+//
+// Ordering: a unit's stores complete to the L2 before its workgroup counts
+// the unit finished; a workgroup acquires at device scope when it takes a
+// unit of another phase than its last one, and before closing a phase. All of
+// a job's workgroups run on its XCD, so the L2 they share carries a phase's
+// results to the next phase. (Dropping only the CU's L1 at the acquire was
+// measured not to be enough: with many workgroups on one job, results then
+// differed from the job run alone.)
+template <bool GUIDED>
+__global__ __launch_bounds__(kWave* kWavesPerWG) __attribute__((amdgpu_waves_per_eu(4))) void k_engine(QEngineArgs e, const BatchArgs* __restrict__ args) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ MapDesc smaps[kMapCache];
+    __shared__ uint32_t s_job, s_ph, s_unit, s_stop, s_last, s_key;
+    constexpr int C = lut_copies<GUIDED>();
+    float* lut = reinterpret_cast<float*>(smem);
+    float* glut = lut + 256 * C;
+    WaveLds* wls = reinterpret_cast<WaveLds*>(smem + lut_bytes(GUIDED));
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    const int lane = threadIdx.x % kWave;
+    // the search's own state lives in LDS: nothing of it is held in registers
+    // across a unit's work
+    __shared__ uint32_t s_hint, s_seen, s_last_job, s_last_ph;
+    __shared__ unsigned long long s_idle_since;
+    int32_t* oc = offtab_lds();
+    for (uint32_t t = threadIdx.x; t < kOffCache && t < e.offtab_n; t += blockDim.x) oc[t] = e.offtab[t];
+    if (threadIdx.x == 0) {
+        const uint32_t x = xcc_id();
+        const uint32_t j0 = e.ctl->job0[x], nj = e.ctl->job0[x + 1] - j0;
+        s_key = 0;
+        s_hint = nj ? j0 + (blockIdx.x / 8u) % nj : j0;
+        s_idle_since = 0;
+        s_seen = 0;
+        s_last_job = s_last_ph = ~0u;
+    }
+    __shared__ unsigned long long s_t[4];
+    __shared__ uint32_t s_nu[2];
+    if (threadIdx.x < 4) s_t[threadIdx.x] = 0;
+    if (threadIdx.x < 2) s_nu[threadIdx.x] = 0;
+    for (;;) {
+        __syncthreads();
+        const uint64_t c0 = threadIdx.x == 0 ? wall_clock64() : 0;
+        if (wave == 0) {
+            const uint32_t xcc = xcc_id();
+            const uint32_t j0 = e.ctl->job0[xcc], nj = e.ctl->job0[xcc + 1] - j0;
+            const uint32_t hint = s_hint;
+            uint64_t idle_since = s_idle_since;
+            uint32_t seen = s_seen;
+            uint32_t gj = ~0u, gph = 0, gu = 0, stop = 0;
+            for (;;) {
+                bool any = false;
+                for (uint32_t base = 0; base < nj && gj == ~0u; base += kWave) {
+                    const uint32_t r = base + (uint32_t)lane;
+                    uint32_t jj = 0;
+                    bool avail = false;
+                    if (r < nj) {
+                        jj = j0 + (hint - j0 + r) % nj;
+                        const unsigned long long c =
+                            __hip_atomic_load(&e.dyn[jj].cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        const uint32_t ph = (uint32_t)(c >> 32);
+                        if (ph < e.jobs[jj].n_phases) avail = (uint32_t)c < e.jobs[jj].phases[ph].units;
+                    }
+                    uint64_t m = __ballot(avail);
+                    while (m) {
+                        any = true;
+                        const int src = __ffsll((unsigned long long)m) - 1;
+                        m &= m - 1;
+                        const uint32_t cj = (uint32_t)__shfl((int)jj, src);
+                        unsigned long long old = 0;
+                        if (lane == 0)
+                            old = __hip_atomic_fetch_add(&e.dyn[cj].cursor, 1ull, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+                        const uint32_t ph = (uint32_t)__shfl((int)(uint32_t)(old >> 32), 0);
+                        const uint32_t u = (uint32_t)__shfl((int)(uint32_t)old, 0);
+                        if (ph < e.jobs[cj].n_phases && u < e.jobs[cj].phases[ph].units) {
+                            gj = cj;
+                            gph = ph;
+                            gu = u;
+                            break;
+                        }
+                    }
+                }
+                if (gj != ~0u) break;
+                // nothing to take: done when every job of the XCD is through
+                if (__hip_atomic_load(&e.ctl->x[xcc].active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+                    stop = 1;
+                    break;
+                }
+                const uint32_t pg = __hip_atomic_load(&e.ctl->x[xcc].progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint64_t now = wall_clock64();
+                if (idle_since == 0 || pg != seen) {
+                    idle_since = now;
+                    seen = pg;
+                } else if (now - idle_since > e.idle_ticks) {
+                    if (lane == 0) atomicOr(&e.ctl->x[xcc].error, 1u);
+                    stop = 1;
+                    break;
+                }
+                if (!any) __builtin_amdgcn_s_sleep(8);
+            }
+            const bool same = gj == s_last_job && gph == s_last_ph;
+            if (lane == 0) {
+                s_job = gj;
+                s_ph = gph;
+                s_unit = gu;
+                s_stop = stop;
+                s_seen = seen;
+                s_idle_since = stop ? idle_since : 0;
+                if (!stop) {
+                    s_hint = gj;
+                    s_last_job = gj;
+                    s_last_ph = gph;
+                }
+            }
+            // a workgroup that stays in one phase has seen its inputs already
+            if (!stop && !same) unit_acquire();
+        }
+        __syncthreads();
+        const uint64_t c1 = threadIdx.x == 0 ? wall_clock64() : 0;
+        if (threadIdx.x == 0) s_t[0] += c1 - c0;
+        if (s_stop) break;
+        // uniform: the job's and the phase's arguments go to scalar registers
+        const uint32_t jb = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_job);
+        const uint32_t phi = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_ph);
+        const uint32_t unit = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_unit);
+        const QJob& J = e.jobs[jb];
+        const QPhase P = J.phases[phi];
+        const uint32_t tid = threadIdx.x;
+        if (P.kind == Q_RESOLVE) {
+            const BatchArgs& a = args[P.arg];
+            // the job stage's cost tables and map descriptors in LDS
+            const uint32_t key = ((jb << 4) | P.stage) + 1u;
+            const bool reload = s_key != key;
+            if (reload) {
+                for (int t = (int)tid; t < 256 * C; t += (int)blockDim.x) {
+                    lut[t] = a.lut[t / C];
+                    if (GUIDED) glut[t] = a.lut[256 + t / C];
+                }
+                if (a.n_maps <= kMapCache && tid < a.n_maps) smaps[tid] = a.maps[tid];
+                __syncthreads();
+            }
+            const MapDesc* maps = a.n_maps <= kMapCache ? smaps : a.maps;
+            const int T = (int)P.team;
+            const int team = wave / T, member = wave % T;
+            const uint32_t item = P.i0 + unit * (uint32_t)(kWavesPerWG / T) + (uint32_t)team;
+            int ln = lane;
+            asm volatile("" : "+v"(ln));
+            resolve_team<GUIDED>(a, maps, item, a.item0, a.n_res, wls + team * T, member, T, item < a.item1, lut,
+                                 glut, ln);
+            if (reload && tid == 0) s_key = key;
+        } else if (P.kind == Q_COMMIT) {
+            const BatchArgs& a = args[P.arg];
+            const uint32_t i = P.i0 + unit * kQElems + tid;
+            if (tid < kQElems && i < P.i1)
+                commit_one(a.W, a.tiles_x, a.maps, a.work, J.result, i, P.i0, a.first_new, a.state_w, a.mask_w,
+                           a.score_w, a.gmask);
+        } else if (P.kind == Q_RAND) {
+            const BatchArgs& a = args[P.arg];
+            const RandArgs ra{a.maps, a.n_maps, a.stage_seed, P.i0, P.i1, a.m, J.rc, J.overflow};
+            const uint32_t i = P.i0 + unit * kQElems + tid;
+            if (tid < kQElems && i < P.i1) rand_item(ra, i);
+        } else if (P.kind == Q_RECOLOR) {
+            const BatchArgs& a = args[P.arg];
+            const uint32_t i = P.i0 + unit * kQElems + tid;
+            if (tid < kQElems && i < P.i1) {
+                const uint32_t p = a.rlist[i];
+                const uint4 st = a.state_w[p];
+                const MapDesc md = a.maps[st.z & 0xffffu];
+                reinterpret_cast<uint32_t*>(&a.state_w[p])[0] = md.px[(st.y >> 16) * md.w + (st.y & 0xffffu)];
+            }
+        } else if (P.kind == Q_STATS) {
+            const BatchArgs& a = args[P.arg];
+            const uint32_t lo = P.i0 + unit * kStatsChunk;
+            const uint32_t hi = min(P.i1, lo + kStatsChunk);
+            double* red = reinterpret_cast<double*>(wls);
+            unsigned long long* redn = reinterpret_cast<unsigned long long*>(red + 256);
+            double sm;
+            unsigned long long cn;
+            stats_chunk(a.item_score, lo, hi, red, redn, sm, cn);
+            if (tid == 0) {
+                J.st_part[unit] = sm;
+                J.st_part_n[unit] = cn;
+            }
+        } else if (P.kind == Q_CLEAR) {
+            const uint32_t t = P.i0 + unit * kQElems + tid;
+            if (tid < kQElems && t < P.i1) {
+                J.mend[t] = 0ull;
+                J.tmin[t] = 0xffffffffu;
+            }
+        } else if (P.kind == Q_MASKOR) {
+            const BatchArgs& a = args[P.arg];
+            const uint32_t q = P.i0 + unit * kQElems + tid;
+            if (tid < kQElems && q < P.i1) {
+                const uint32_t p = a.rlist[q], px = p % a.W, py = p / a.W;
+                const uint32_t t = (py >> 3) * a.tiles_x + (px >> 3);
+                atomicOr((unsigned long long*)&J.mend[t], 1ull << (((py & 7u) << 3) | (px & 7u)));
+                atomicMin(&J.tmin[t], q);
+            }
+        } else if (P.kind == Q_PREFETCH) {
+            const BatchArgs& a = args[P.arg];
+            PrefetchArgs pa{};
+            pa.W = a.W;
+            pa.H = a.H;
+            pa.tiles_x = a.tiles_x;
+            pa.tiles_y = a.tiles_y;
+            pa.mask = a.mask;
+            if (P.pre_end) {
+                pa.mask_new = J.mend;
+                pa.tile_min = J.tmin;
+                pa.rank = J.rank;
+            }
+            pa.rlist = a.rlist;
+            pa.work = a.work;
+            pa.n_at = J.n_at + P.pre_base;
+            pa.offtab = a.offtab;
+            pa.offtab_n = a.offtab_n;
+            pa.k = a.k;
+            pa.item0 = P.i0;
+            pa.item1 = P.i1;
+            pa.out_nbrs = J.pre_nbrs;
+            pa.out_cnt = J.pre_cnt;
+            const uint32_t item = P.i0 + unit * kWavesPerWG + (uint32_t)wave;
+            if (item < P.i1) prefetch_item(pa, wls + wave, item, lane);
+        }
+        // ---- the unit is done
+        unit_release();
+        __syncthreads();
+        if (tid == 0) {
+            const uint64_t c2 = wall_clock64();
+            s_t[P.kind == Q_RESOLVE ? 1 : 2] += c2 - c1;
+            s_nu[P.kind == Q_RESOLVE ? 0 : 1] += 1;
+            const uint32_t old = __hip_atomic_fetch_add(&e.dyn[jb].finished, 1u, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&e.ctl->x[xcc_id()].progress, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_last = old + 1u == P.units;
+            if (s_last) unit_acquire();
+        }
+        __syncthreads();
+        if (s_last) {
+            const uint64_t c3 = tid == 0 ? wall_clock64() : 0;
+            // closing work of the phase: every unit's results are in the L2
+            if (P.kind == Q_RESOLVE && P.fuse) {
+                const BatchArgs& a = args[P.arg];
+                for (uint32_t i = P.i0 + tid; i < P.i1; i += blockDim.x)
+                    commit_one(a.W, a.tiles_x, a.maps, a.work, J.result, i, P.i0, a.first_new, a.state_w, a.mask_w,
+                               a.score_w, a.gmask);
+            } else if (P.kind == Q_STATS && tid == 0) {
+                stats_final(J.st_part, J.st_part_n, P.units, J.st_sum + P.stage, J.st_cnt + P.stage);
+            }
+            unit_release();
+            __syncthreads();
+            if (tid == 0) {
+                __hip_atomic_store(&e.dyn[jb].finished, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                unit_release();
+                __hip_atomic_store(&e.dyn[jb].cursor, (unsigned long long)(phi + 1u) << 32, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                if (phi + 1u == J.n_phases) atomicSub(&e.ctl->x[xcc_id()].active, 1u);
+                s_t[3] += wall_clock64() - c3;
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        QXcd& X = e.ctl->x[xcc_id()];
+        atomicAdd(&X.t_find, s_t[0]);
+        atomicAdd(&X.t_resolve, s_t[1]);
+        atomicAdd(&X.t_other, s_t[2]);
+        atomicAdd(&X.t_close, s_t[3]);
+        atomicAdd(&X.units_resolve, s_nu[0]);
+        atomicAdd(&X.units_other, s_nu[1]);
+    }
+}
+
 }  // namespace
 
 namespace {
@@ -1938,6 +2326,11 @@ void set_launch_events(hipEvent_t start, hipEvent_t stop) {
 }
 
 hipError_t configure_kernels() {
+    for (const void* f : {(const void*)k_engine<true>, (const void*)k_engine<false>}) {
+        const hipError_t e0 = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                  (int)resolve_lds_bytes(f == (const void*)k_engine<true>));
+        if (e0 != hipSuccess) return e0;
+    }
     hipError_t e = hipFuncSetAttribute((const void*)k_resolve<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)resolve_lds_bytes(true));
     if (e != hipSuccess) return e;
@@ -2040,6 +2433,48 @@ hipError_t launch_knn_unpack(const uint32_t* q_flat, const int32_t* packed, cons
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_knn_unpack, dim3((n + 255) / 256), dim3(256), 0, s, q_flat, packed, counts, n_q, k, W,
                        pairs);
+    return hipGetLastError();
+}
+
+hipError_t launch_stats_chunks(const float* item_score, uint32_t n, double* part, unsigned long long* part_n,
+                               double* sum, unsigned long long* count, hipStream_t s) {
+    const uint32_t parts = (n + kStatsChunk - 1) / kStatsChunk;
+    if (parts > 0) hipLaunchKernelGGL(k_stats_part, dim3(parts), dim3(256), 0, s, item_score, n, part, part_n);
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, s, part, part_n, parts, sum, count);
+    return hipGetLastError();
+}
+
+uint32_t engine_max_wgs(bool guided) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    const void* f = guided ? (const void*)k_engine<true> : (const void*)k_engine<false>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kWave * kWavesPerWG, resolve_lds_bytes(guided)) !=
+        hipSuccess)
+        return 0;
+    return (uint32_t)(per_cu * cus);
+}
+
+hipError_t launch_engine(const QEngineArgs& e, const BatchArgs* args, uint32_t wgs, bool guided, hipStream_t s) {
+    if (wgs == 0 || !e.jobs || !e.dyn || !e.ctl || !args || e.idle_ticks == 0) return hipErrorInvalidValue;
+    const size_t lds = resolve_lds_bytes(guided);
+    if (guided) {
+        hipLaunchKernelGGL(k_engine<true>, dim3(wgs), dim3(kWave * kWavesPerWG), lds, s, e, args);
+    } else {
+        hipLaunchKernelGGL(k_engine<false>, dim3(wgs), dim3(kWave * kWavesPerWG), lds, s, e, args);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_exp_sweep(uint64_t i0, uint64_t n, float* out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_exp_sweep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, i0, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_xcc_probe(uint32_t* out, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_xcc_probe, dim3(n), dim3(64), 0, s, out, n);
     return hipGetLastError();
 }
 

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/tsgpu.h"
+#include "exp_dd.h"
 #include "kernels.h"
 
 using namespace tsg;
@@ -81,8 +82,14 @@ DevPool* pool_here() {
     return &g_pool[d];
 }
 
+// Test hook (tsg_debug_fail_allocs): the next this-many device allocations
+// of the library fail as if the device were out of memory.
+std::atomic<int64_t> g_fail_allocs{0};
+
 // A cached block of at least `bytes` (and at most twice that), or a new one.
 hipError_t pool_get(size_t bytes, void** p, size_t* got) {
+    if (g_fail_allocs.load(std::memory_order_relaxed) > 0 && g_fail_allocs.fetch_sub(1) > 0)
+        return hipErrorOutOfMemory;
     if (DevPool* pl = pool_here()) {
         std::lock_guard<std::mutex> lk(pl->mu);
         auto it = pl->free.lower_bound(bytes);
@@ -450,11 +457,14 @@ struct Staging {
 
 struct tsg_job;
 namespace {
-void picks_work(tsg_job* j);
+bool picks_step(tsg_job* j);
+void picks_requeued(tsg_job* j);
 }  // namespace
 
 // Host threads that compute jobs' pick orders (the reference's swap_remove
 // chain over the unresolved list) while the device works on earlier stages.
+// A worker computes one stage of a job, then puts the job at the back of the
+// queue: with many jobs queued, every job's first stage is ready early.
 struct PicksPool {
     std::mutex mu;
     std::condition_variable cv;
@@ -491,7 +501,12 @@ struct PicksPool {
                 j = queue.front();
                 queue.pop_front();
             }
-            picks_work(j);
+            if (picks_step(j)) {
+                std::lock_guard<std::mutex> lk(mu);
+                picks_requeued(j);
+                queue.push_back(j);
+                cv.notify_one();
+            }
         }
     }
     ~PicksPool() {
@@ -625,6 +640,19 @@ struct tsg_job {
     DBuf<uint32_t> overflow;
     DBuf<double> st_sum;
     DBuf<unsigned long long> st_cnt;
+    DBuf<double> st_part;                // per kStatsChunk items of a stage: partial sums and counts
+    DBuf<unsigned long long> st_part_n;
+    // under the work-queue engine: what was planned since the last launch
+    struct Eng {
+        std::vector<QPhase> prog;        // phases (arg: index into args)
+        std::vector<BatchArgs> args;     // resolve phases' arguments and the open stages' entries
+        std::vector<uint32_t> n_at;      // prefetch windows' resolved counts (pre_base: offset)
+        uint32_t stage_arg = 0;          // the open stage's entry in args
+        uint32_t sharing = 1;            // jobs sharing the device when planned
+        uint64_t weight = 0;             // work of the planned phases (XCD placement)
+        bool planned_all = false;        // every phase of the job is planned
+        DBuf<float> luts;                // per stage: 512-entry cost tables
+    } eng;
     DBuf<uint32_t> done_counter;
     DBuf<uint32_t> small_tab;  // batch bounds and resolved counts of small runs
     // neighbour lists computed ahead of latency-bound batches
@@ -926,44 +954,47 @@ constexpr uint64_t kAheadBudget = 16ull << 30;  // device bytes of the stage-ahe
 
 // ---- resumable stage driver (Generator::resolve, ms.rs:786-1050) ----
 
-// Pick orders of every stage's new pixels, in stage order (the chain depends
-// only on counts and seeds, not on image content). Runs on a pool thread;
-// each stage is published as soon as it is done.
-void picks_work(tsg_job* j) {
+// Pick order of the next stage's new pixels (the chain depends only on
+// counts and seeds, not on image content), published as soon as it is done.
+// Runs on a pool thread; true when stages remain (the pool queues the job
+// again), false when every stage is done or the job was cancelled.
+bool picks_step(tsg_job* j) {
+    const int P = (int)j->prm.backtrack_stages;
+    int t = 0;
+    uint64_t planned = j->pk_start_res;
     {
         std::lock_guard<std::mutex> lk(j->pk_mu);
-        j->pk_state = 2;
-    }
-    const int P = (int)j->prm.backtrack_stages;
-    uint64_t planned = j->pk_start_res;
-    for (int t = 0; t <= P; ++t) {
-        {
-            std::lock_guard<std::mutex> lk(j->pk_mu);
-            if (j->pk_cancel) break;
-            if (j->picks_ready[t]) {
-                planned += j->picks_keep[t].size();
-                continue;
-            }
+        if (j->pk_cancel) {
+            j->pk_state = 3;
+            j->pk_cv.notify_all();
+            return false;
         }
+        j->pk_state = 2;
+        while (t <= P && j->picks_ready[t]) planned += j->picks_keep[t++].size();
+    }
+    if (t <= P) {
         const int ps = P - t;
         const uint64_t pss = (uint64_t)Pcg32::seed_from_u64(j->prm.seed + (uint64_t)ps).next_u32();
         const uint64_t S = stage_size(j, ps);
         const uint64_t redo = planned - j->locked;
         std::vector<uint32_t> picks;
         if (S > redo) picks = pick_new(j, pss, redo, S - redo);
-        planned += picks.size();
-        {
-            std::lock_guard<std::mutex> lk(j->pk_mu);
-            j->picks_keep[t] = std::move(picks);
-            j->picks_ready[t] = 1;
-        }
-        j->pk_cv.notify_all();
-    }
-    {
         std::lock_guard<std::mutex> lk(j->pk_mu);
-        j->pk_state = 3;
+        j->picks_keep[t] = std::move(picks);
+        j->picks_ready[t] = 1;
+        ++t;
     }
+    std::lock_guard<std::mutex> lk(j->pk_mu);
+    const bool more = t <= P && !j->pk_cancel;
+    if (!more) j->pk_state = 3;
     j->pk_cv.notify_all();
+    return more;
+}
+
+// The job goes back into the pool's queue for its next stage.
+void picks_requeued(tsg_job* j) {
+    std::lock_guard<std::mutex> lk(j->pk_mu);
+    j->pk_state = 1;
 }
 
 bool picks_ready_now(tsg_job* j, int t) {
@@ -1410,8 +1441,10 @@ void step_advance(tsg_job* j, uint64_t i0, uint64_t i1) {
 
 int step_close_stage(tsg_job* j) {
     tsg_job::Step& st = j->step;
-    StatsArgs sa{j->item_score.p, (uint32_t)st.S_eff, j->st_sum.p + st.t, j->st_cnt.p + st.t};
-    HIP_TRY(timed_launch(j, K_STATS, st.S_eff, [&] { return launch_stats(sa, j->ctx->stream); }));
+    HIP_TRY(timed_launch(j, K_STATS, st.S_eff, [&] {
+        return launch_stats_chunks(j->item_score.p, (uint32_t)st.S_eff, j->st_part.p, j->st_part_n.p,
+                                   j->st_sum.p + st.t, j->st_cnt.p + st.t, j->ctx->stream);
+    }));
     if (j->ahead_on) {
         tsg_job::Ahead& A = j->aset(st.t);
         if (!A.done) HIP_TRY(hipEventCreateWithFlags(&A.done, hipEventDisableTiming));
@@ -1812,6 +1845,7 @@ size_t job_footprint(const tsg_job* j) {
              A.tmin.n * 4 + A.dep.n * 8;
     b += j->flow_flag.n * 4 + j->flow_ticket.n * 4 + j->state_old.n * sizeof(uint4);
     b += j->state2.n * sizeof(uint4) + j->stamp.n * 4 + j->pend.n * 8;
+    b += j->st_part.n * 8 + j->st_part_n.n * 8 + j->eng.luts.n * 4;
     return b;
 }
 
@@ -1824,187 +1858,476 @@ struct tsg_batch {
         uint64_t ticket;
         tsg_job* job;
         int status;
+        bool failed;  // status set by the engine (planning error, cancellation)
     };
     std::vector<Entry> active, pending, done, taken;
     uint64_t next_ticket = 1;
-    static constexpr int kRing = 8;
-    // per-round tables, one pinned block per ring slot copied in one piece:
-    // [starts: 5 x (cap + 2) u32 | random-candidate units | batch units]
-    char* h_blk[kRing] = {};
-    DBuf<char> d_blk[kRing];
-    size_t off_rand = 0, off_units = 0;
-    hipEvent_t ring_ev[kRing] = {};
-    bool ring_used[kRing] = {};
-    uint32_t cap_units = 0;
-    int ring = 0;
+    // one launch's tables, built in pinned memory and copied in one piece:
+    // [QCtl x 2 | QJob | QDyn | QPhase | BatchArgs | n_at]
+    char* h_blk = nullptr;
+    size_t h_cap = 0;
+    DBuf<char> d_blk;
+    uint32_t wgs[2] = {0, 0};  // resident engine workgroups (unguided, guided)
     uint64_t rounds = 0, launches = 0, units_total = 0;
+    uint64_t idle_ticks = 200000000ull;  // 2 s of the 100 MHz clock without progress on an XCD
+    uint32_t wgs_cap = 0;                 // test hook: at most this many engine workgroups (0: all resident)
+    double prof[16] = {};                 // accumulated QXcd clocks and counts (tsg_batch_engine_profile)
+    uint32_t engine_flags = 0;            // QEngineArgs::flags
     tsg_tuning tune{};
     ~tsg_batch() {
-        for (int r = 0; r < kRing; ++r) {
-            if (h_blk[r]) (void)hipHostFree(h_blk[r]);
-            if (ring_ev[r]) (void)hipEventDestroy(ring_ev[r]);
-        }
+        if (h_blk) (void)hipHostFree(h_blk);
     }
 };
 
 namespace {
 
+// Phases of at most this many items commit in the batch's closing workgroup.
+constexpr uint32_t kEngineFuseMax = 2048;
+// Launches with at most this many jobs compute the neighbour lists of
+// latency-bound batches ahead, as a single-job run does.
+constexpr uint32_t kEnginePrefetchJobs = 8;
+constexpr uint32_t kNoArg = 0xffffffffu;
+
+void eng_push(tsg_job* j, uint32_t kind, uint32_t i0, uint32_t i1, uint32_t per_unit, uint32_t arg) {
+    QPhase q{};
+    q.kind = kind;
+    q.stage = (uint32_t)j->step.t;
+    q.i0 = i0;
+    q.i1 = i1;
+    q.units = (i1 - i0 + per_unit - 1) / per_unit;
+    q.arg = arg == kNoArg ? j->eng.stage_arg : arg;
+    if (q.units) j->eng.prog.push_back(q);
+}
+
+// Stage preamble (ms.rs:786-858) as planned work: the recolour phase, the
+// stage's cost tables and picks uploaded, the stage's batch arguments.
+int eng_open_stage(tsg_job* j) {
+    hipStream_t s = j->ctx->stream;
+    tsg_job::Step& st = j->step;
+    const int P = (int)j->prm.backtrack_stages;
+    const int t = st.t;
+    const int ps = P - t;
+    const uint32_t nf = (uint32_t)j->filt.size();
+    st.level = t == 0 ? 0u : (uint32_t)std::min(t, P - 1);
+    const MapDesc* maps = j->maps.p + (size_t)st.level * nf;
+    st.pss = (uint64_t)Pcg32::seed_from_u64(j->prm.seed + (uint64_t)ps).next_u32();
+    st.S = stage_size(j, ps);
+    st.redo = j->n_res - j->locked;
+    float alpha_s = 0.0f;
+    if (j->guided && ps > 0) {
+        float b = j->prm.guide_alpha * (1.0f - ((float)j->n_res / (float)j->N0));
+        alpha_s = (b * b) * b;
+    }
+    stage_luts(j->prm.cauchy_dispersion, j->guided, alpha_s, j->lut_keep[t].data());
+    uint32_t lut_nan = 0;
+    for (float v : j->lut_keep[t]) lut_nan |= (uint32_t)std::isnan(v);
+    float* lut = j->eng.luts.p + (size_t)t * 512;
+    HIP_TRY(j->ctx->staging.upload(lut, j->lut_keep[t].data(), 512 * 4, s));
+    int rc = wait_picks(j, t);
+    if (rc) return rc;
+    st.S_eff = st.S;
+    if (st.S > st.redo) {
+        st.S_eff = st.redo + j->picks_keep[t].size();
+        if (!j->picks_keep[t].empty()) {
+            HIP_TRY(j->ctx->staging.upload(j->rlist.p + j->locked + st.redo, j->picks_keep[t].data(),
+                                           j->picks_keep[t].size() * 4, s));
+            const uint32_t lo = (uint32_t)(j->locked + st.redo);
+            HIP_TRY(launch_rank(j->rlist.p, lo, lo + (uint32_t)j->picks_keep[t].size(), j->rank.p, s));
+        }
+    }
+    st.redo_eff = std::min(st.redo, st.S_eff);
+    st.i = 0;
+    st.rc_lo = st.rc_hi = 0;
+    BatchArgs& a = st.a;
+    a = BatchArgs{};
+    a.W = j->W;
+    a.H = j->H;
+    a.tiles_x = j->tiles_x;
+    a.tiles_y = j->tiles_y;
+    a.mask = j->mask.p;
+    a.state = j->state.p;
+    a.tg = j->guided ? j->tg.p + (size_t)st.level * j->N : nullptr;
+    a.maps = maps;
+    a.n_maps = nf;
+    a.work = j->rlist.p + j->locked;
+    a.rlist = j->rlist.p;
+    a.m = (uint32_t)j->prm.random_sample_locations;
+    a.offtab = j->ctx->offtab.p;
+    a.offtab_n = j->ctx->offtab_n;
+    a.lut = lut;
+    a.stage_seed = st.pss;
+    a.k = j->prm.nearest_neighbors;
+    a.item_score = j->item_score.p;
+    a.lut_nan = lut_nan;
+    a.first_new = (uint32_t)st.redo_eff;
+    a.state_w = j->state.p;
+    a.mask_w = j->mask.p;
+    a.score_w = j->score.p;
+    a.result = j->result.p;
+    a.rcands = j->rc.p;
+    if (j->prm.tiling_mode) {
+        a.gmask = j->gmask.p;
+        a.xl = (int32_t)((float)j->W * 0.05f);
+        a.xr = (int32_t)j->W - a.xl;
+        a.yb = (int32_t)((float)j->H * 0.05f);
+        a.yt = (int32_t)j->H - a.yb;
+    }
+    j->eng.stage_arg = (uint32_t)j->eng.args.size();
+    j->eng.args.push_back(a);
+    // next_pyramid_level (ms.rs:687-700): every resolved pixel from the new level
+    if (st.level > 0) eng_push(j, Q_RECOLOR, 0, (uint32_t)j->n_res, kQElems, kNoArg);
+    st.open = true;
+    return TSG_OK;
+}
+
+// The prefetch window that starts at i0 (as step_prefetch plans it), as
+// phases: the window's new pixels into the window masks, then one wave per
+// item's neighbour list.
+void eng_prefetch(tsg_job* j, uint64_t i0) {
+    tsg_job::Step& st = j->step;
+    const bool redo = i0 < st.redo_eff;
+    const uint64_t seg_end = redo ? st.redo_eff : st.S_eff;
+    const uint32_t off = (uint32_t)j->eng.n_at.size();
+    uint64_t i = i0, nres = j->n_res;
+    while (i < seg_end) {
+        const uint64_t e = batch_end_at(j, st.t, i, nres, st.redo, st.S_eff);
+        if (e - i > j->tune.prefetch_max || e - i0 > kPrefetchCap) break;
+        j->eng.n_at.insert(j->eng.n_at.end(), e - i, (uint32_t)nres);
+        if (e > st.redo_eff) nres += e - std::max(i, st.redo_eff);
+        i = e;
+    }
+    const uint64_t w1 = i;
+    if (w1 <= i0) return;
+    if (!redo) {
+        eng_push(j, Q_CLEAR, 0, j->tiles_x * j->tiles_y, kQElems, kNoArg);
+        eng_push(j, Q_MASKOR, (uint32_t)(j->locked + i0), (uint32_t)(j->locked + w1), kQElems, kNoArg);
+    }
+    eng_push(j, Q_PREFETCH, (uint32_t)i0, (uint32_t)w1, 8, kNoArg);
+    j->eng.prog.back().pre_base = off;
+    j->eng.prog.back().pre_end = redo ? 0u : 1u;
+    st.a.pre_nbrs = j->pre_nbrs.p;
+    st.a.pre_cnt = j->pre_cnt.p;
+    st.a.pre_base = (uint32_t)i0;
+    st.a.pre_end = (uint32_t)w1;
+}
+
+// Plans the job as far as its pick orders allow: the same batches, random
+// candidate chunks and prefetch windows as a run alone, as phases of the
+// next launch.
+int eng_plan(tsg_job* j) {
+    tsg_job::Step& st = j->step;
+    int rc;
+    while (!step_done(j)) {
+        if (!st.open) {
+            if (!picks_ready_now(j, st.t)) return TSG_OK;
+            if ((rc = eng_open_stage(j))) return rc;
+        }
+        uint64_t i0 = 0, i1 = 0;
+        if (!step_peek(j, i0, i1)) {
+            // stage statistics, then the stage is through (step_close_stage)
+            eng_push(j, Q_STATS, 0, (uint32_t)st.S_eff, kStatsChunk, kNoArg);
+            j->st_items[st.t] = st.S_eff;
+            j->overall_cur += st.S;
+            st.open = false;
+            st.t += 1;
+            continue;
+        }
+        const uint32_t m = (uint32_t)j->prm.random_sample_locations;
+        if (!(i0 >= st.rc_lo && i1 <= st.rc_hi)) {
+            const uint64_t cap_items = std::max<uint64_t>(i1 - i0, (uint64_t)j->rc_budget / (8ull * std::max(1u, m)));
+            st.rc_lo = i0;
+            st.rc_hi = std::min<uint64_t>(st.S_eff, i0 + cap_items);
+            if ((st.rc_hi - st.rc_lo) * m > j->rc.n)
+                return fail(TSG_ERR_DEVICE, "random-candidate chunk beyond its buffer");
+            eng_push(j, Q_RAND, (uint32_t)st.rc_lo, (uint32_t)st.rc_hi, kQElems, kNoArg);
+        }
+        // neighbour lists ahead of latency-bound batches shorten a lone job's
+        // chain; with other jobs to run meanwhile the resolve searches itself
+        if (j->eng.sharing <= kEnginePrefetchJobs && step_wants_prefetch(j, i0, i1)) eng_prefetch(j, i0);
+        BatchArgs a = st.a;
+        a.n_res = (uint32_t)j->n_res;
+        a.item0 = (uint32_t)i0;
+        a.item1 = (uint32_t)i1;
+        a.rc_base = (uint32_t)st.rc_lo;
+        // teams of waves per item shorten a batch's latency at the cost of
+        // throughput; the jobs of a launch share the device, so the team is
+        // sized against the waves per job
+        a.team = team_for((i1 - i0) * j->eng.sharing, j->tune.team_max, j->tune.team_waves);
+        if (!(a.pre_nbrs && i0 >= a.pre_base && i1 <= a.pre_end)) {
+            a.pre_nbrs = nullptr;
+            a.pre_cnt = nullptr;
+            a.pre_base = a.pre_end = 0;
+        }
+        const uint32_t arg = (uint32_t)j->eng.args.size();
+        j->eng.args.push_back(a);
+        const uint32_t n = (uint32_t)(i1 - i0);
+        eng_push(j, Q_RESOLVE, (uint32_t)i0, (uint32_t)i1, items_per_wg(a.team), arg);
+        QPhase& q = j->eng.prog.back();
+        q.n_res = a.n_res;
+        q.team = a.team;
+        q.rc_base = a.rc_base;
+        q.pre_base = a.pre_base;
+        q.pre_end = a.pre_end;
+        q.fuse = n <= kEngineFuseMax;
+        if (!q.fuse) eng_push(j, Q_COMMIT, (uint32_t)i0, (uint32_t)i1, kQElems, kNoArg);
+        j->eng.weight += (uint64_t)n * (j->prm.nearest_neighbors + 8);
+        step_advance(j, i0, i1);
+    }
+    j->eng.planned_all = true;
+    return TSG_OK;
+}
+
+// Engine buffers of an admitted job, sized for its largest batch, chunk and window.
+int eng_begin(tsg_job* j) {
+    int rc = step_begin(j);
+    if (rc) return rc;
+    const int P = (int)j->prm.backtrack_stages;
+    HIP_TRY(j->eng.luts.reserve((size_t)(P + 1) * 512));
+    const uint64_t m = std::max<uint64_t>(1, j->prm.random_sample_locations);
+    const uint64_t maxb = j->result.n;
+    HIP_TRY(j->rc.reserve(std::min<uint64_t>(j->N, std::max<uint64_t>(maxb, j->rc_budget / (8 * m))) * m));
+    const uint64_t win = std::min<uint64_t>(kPrefetchCap, j->N);
+    HIP_TRY(j->pre_cnt.reserve(win));
+    HIP_TRY(j->pre_nbrs.reserve(win * j->prm.nearest_neighbors));
+    const size_t tiles = (size_t)j->tiles_x * j->tiles_y;
+    HIP_TRY(j->mend.reserve(tiles));
+    HIP_TRY(j->tmin.reserve(tiles));
+    j->eng.prog.clear();
+    j->eng.args.clear();
+    j->eng.n_at.clear();
+    j->eng.stage_arg = 0;
+    j->eng.weight = 0;
+    j->eng.planned_all = false;
+    return TSG_OK;
+}
+
+// A job leaves the engine with a status of its own; its queued uploads (if
+// any) finish before the end event.
+int eng_drop(tsg_batch* b, size_t q, int status) {
+    tsg_batch::Entry e = b->active[q];
+    b->active.erase(b->active.begin() + q);
+    e.status = status;
+    e.failed = true;
+    tsg_job* j = e.job;
+    if (j->ev_run1) (void)hipEventRecord(j->ev_run1, b->ctx->stream);
+    j->eng.prog.clear();
+    j->eng.args.clear();
+    j->eng.n_at.clear();
+    b->done.push_back(e);
+    return TSG_OK;
+}
+
 int batch_admit(tsg_batch* b) {
-    std::vector<tsg_job*> adm;
     while (b->active.size() < b->max_active && !b->pending.empty()) {
         tsg_batch::Entry e = b->pending.front();
         b->pending.erase(b->pending.begin());
         b->active.push_back(e);
-        adm.push_back(e.job);
-    }
-    if (adm.empty()) return TSG_OK;
-    // pick orders are computed by the context's pool; rounds skip a job
-    // until its next stage's order is ready
-    for (tsg_job* j : adm) {
-        int rc = step_begin(j);
-        if (rc) return rc;
+        const int rc = eng_begin(e.job);
+        if (rc) eng_drop(b, b->active.size() - 1, rc);
     }
     return TSG_OK;
 }
 
-// One round: every active job contributes its next batch.
+size_t up256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// One launch of the engine over everything planned: jobs are placed on the
+// XCDs by work (largest first onto the least loaded), their programs and
+// batch arguments go to the device in one copy, then the unguided and the
+// guided engine kernels run.
 int batch_round(tsg_batch* b) {
     hipStream_t s = b->ctx->stream;
-    int rc;
-    if ((rc = batch_admit(b))) return rc;
+    batch_admit(b);
     if (b->active.empty()) return TSG_OK;
-    struct U {
-        tsg_job* j;
-        uint64_t i0, i1;
-    };
-    std::vector<U> units[2];
-    std::vector<RandArgs> rands;
-    tsg_job* waiter = nullptr;  // a job whose next pick order is not ready
+    // plan every job as far as its picks allow; a job that cannot be planned leaves
     for (size_t q = 0; q < b->active.size();) {
-        tsg_job* j = b->active[q].job;
-        uint64_t i0 = 0, i1 = 0;
-        bool got = false, waiting = false;
-        while (!step_done(j)) {
-            if (!j->step.open) {
-                if (!picks_ready_now(j, j->step.t)) {
-                    waiting = true;
-                    break;
-                }
-                if ((rc = step_open_stage(j))) return rc;
-            }
-            if (step_peek(j, i0, i1)) {
-                got = true;
-                break;
-            }
-            if ((rc = step_close_stage(j))) return rc;
-        }
-        if (waiting) {
-            waiter = j;
-            ++q;
+        b->active[q].job->eng.sharing = (uint32_t)b->active.size();
+        const int rc = eng_plan(b->active[q].job);
+        if (rc) {
+            eng_drop(b, q, rc);
             continue;
         }
-        if (!got) {
-            if ((rc = step_end(j))) return rc;
+        ++q;
+    }
+    std::vector<size_t> runs;
+    for (size_t q = 0; q < b->active.size(); ++q)
+        if (!b->active[q].job->eng.prog.empty()) runs.push_back(q);
+    if (runs.empty()) {
+        // nothing planned: wait for one job's next pick order
+        for (const tsg_batch::Entry& e : b->active)
+            if (!e.job->eng.planned_all) {
+                const int rc = wait_picks(e.job, e.job->step.t);
+                if (rc) return rc;
+                break;
+            }
+    } else {
+        // placement: per kind (unguided 0, guided 1), largest work first onto the least loaded XCD
+        std::vector<size_t> order[2];
+        for (size_t q : runs) order[b->active[q].job->guided ? 1 : 0].push_back(q);
+        std::vector<uint32_t> xcc_of(b->active.size(), 0);
+        for (int kd = 0; kd < 2; ++kd) {
+            std::stable_sort(order[kd].begin(), order[kd].end(), [&](size_t x, size_t y) {
+                return b->active[x].job->eng.weight > b->active[y].job->eng.weight;
+            });
+            // XCDs with engine workgroups: workgroup w runs on XCD w % 8
+            if (!b->wgs[kd]) b->wgs[kd] = engine_max_wgs(kd == 1);
+            const uint32_t wgs_kd = b->wgs_cap ? std::min(b->wgs_cap, b->wgs[kd]) : b->wgs[kd];
+            const int n_x = (int)std::min<uint32_t>(8, std::max<uint32_t>(1, wgs_kd));
+            uint64_t load[8] = {};
+            for (size_t q : order[kd]) {
+                const int x = (int)(std::min_element(load, load + n_x) - load);
+                load[x] += b->active[q].job->eng.weight + 1;
+                xcc_of[q] = (uint32_t)x;
+            }
+            std::stable_sort(order[kd].begin(), order[kd].end(), [&](size_t x, size_t y) { return xcc_of[x] < xcc_of[y]; });
+        }
+        // sizes of the launch block
+        size_t n_jobs = runs.size(), n_ph = 0, n_args = 0, n_at = 0;
+        for (size_t q : runs) {
+            const tsg_job::Eng& E = b->active[q].job->eng;
+            n_ph += E.prog.size();
+            n_args += E.args.size();
+            n_at += E.n_at.size();
+        }
+        const size_t o_ctl = 0, o_job = up256(2 * sizeof(QCtl)), o_dyn = up256(o_job + n_jobs * sizeof(QJob));
+        const size_t o_ph = up256(o_dyn + n_jobs * sizeof(QDyn)), o_args = up256(o_ph + n_ph * sizeof(QPhase));
+        const size_t o_nat = up256(o_args + n_args * sizeof(BatchArgs)), bytes = up256(o_nat + n_at * 4 + 4);
+        if (bytes > b->h_cap) {
+            // grown between launches (nothing reads the old block any more)
+            const size_t cap = std::max(bytes, 2 * b->h_cap);
+            char* h = nullptr;
+            HIP_TRY(hipHostMalloc((void**)&h, cap));
+            DBuf<char> d;
+            const hipError_t e = d.alloc(cap);
+            if (e != hipSuccess) {
+                (void)hipHostFree(h);
+                HIP_TRY(e);
+            }
+            if (b->h_blk) (void)hipHostFree(b->h_blk);
+            b->h_blk = h;
+            b->h_cap = cap;
+            std::swap(b->d_blk.p, d.p);
+            std::swap(b->d_blk.n, d.n);
+            std::swap(b->d_blk.bytes, d.bytes);
+        }
+        char* H = b->h_blk;
+        char* D = b->d_blk.p;
+        memset(H, 0, o_ph);
+        QCtl* ctl = reinterpret_cast<QCtl*>(H + o_ctl);
+        QJob* jobs = reinterpret_cast<QJob*>(H + o_job);
+        QPhase* ph = reinterpret_cast<QPhase*>(H + o_ph);
+        BatchArgs* args = reinterpret_cast<BatchArgs*>(H + o_args);
+        uint32_t* nat = reinterpret_cast<uint32_t*>(H + o_nat);
+        size_t jx = 0, px = 0, ax = 0, nx = 0;
+        size_t first[2] = {0, 0}, count[2] = {0, 0};
+        for (int kd = 0; kd < 2; ++kd) {
+            first[kd] = jx;
+            QCtl& C = ctl[kd];
+            for (size_t q : order[kd]) {
+                tsg_job* j = b->active[q].job;
+                tsg_job::Eng& E = j->eng;
+                const uint32_t x = xcc_of[q];
+                QJob& J = jobs[jx];
+                J.phases = reinterpret_cast<const QPhase*>(D + o_ph) + px;
+                J.n_phases = (uint32_t)E.prog.size();
+                J.xcc = x;
+                J.rc = j->rc.p;
+                J.result = j->result.p;
+                J.overflow = j->overflow.p;
+                J.mend = j->mend.p;
+                J.tmin = j->tmin.p;
+                J.n_at = reinterpret_cast<const uint32_t*>(D + o_nat) + nx;
+                J.pre_nbrs = j->pre_nbrs.p;
+                J.pre_cnt = j->pre_cnt.p;
+                J.rank = j->rank.p;
+                J.st_part = j->st_part.p;
+                J.st_part_n = j->st_part_n.p;
+                J.st_sum = j->st_sum.p;
+                J.st_cnt = j->st_cnt.p;
+                for (const QPhase& p0 : E.prog) {
+                    QPhase p1 = p0;
+                    p1.arg += (uint32_t)ax;
+                    ph[px++] = p1;
+                }
+                memcpy(args + ax, E.args.data(), E.args.size() * sizeof(BatchArgs));
+                ax += E.args.size();
+                if (!E.n_at.empty()) memcpy(nat + nx, E.n_at.data(), E.n_at.size() * 4);
+                nx += E.n_at.size();
+                C.x[x].active += 1;
+                for (uint32_t y = x + 1; y <= 8; ++y) C.job0[y] += 1;
+                b->units_total += E.prog.size();
+                ++jx;
+            }
+            count[kd] = jx - first[kd];
+            for (uint32_t y = 0; y <= 8; ++y) C.job0[y] += (uint32_t)first[kd];
+        }
+        HIP_TRY(hipMemcpyAsync(D, H, o_nat + n_at * 4, hipMemcpyHostToDevice, s));
+        for (int kd = 0; kd < 2; ++kd) {
+            if (!count[kd]) continue;
+            if (!b->wgs[kd]) return fail(TSG_ERR_DEVICE, "engine occupancy query failed");
+            QEngineArgs ea{};
+            ea.jobs = reinterpret_cast<const QJob*>(D + o_job);
+            ea.dyn = reinterpret_cast<QDyn*>(D + o_dyn);
+            ea.ctl = reinterpret_cast<QCtl*>(D + o_ctl) + kd;
+            ea.offtab = b->ctx->offtab.p;
+            ea.offtab_n = b->ctx->offtab_n;
+            ea.idle_ticks = b->idle_ticks;
+            ea.flags = b->engine_flags;
+            const uint32_t wgs = b->wgs_cap ? std::min(b->wgs_cap, b->wgs[kd]) : b->wgs[kd];
+            HIP_TRY(launch_engine(ea, reinterpret_cast<const BatchArgs*>(D + o_args), wgs, kd == 1, s));
+            b->launches += 1;
+        }
+        // control words and every job's cursor: each program must have run to its end
+        std::vector<char> back(o_ph - o_ctl);
+        HIP_TRY(hipMemcpyAsync(back.data(), D + o_ctl, back.size(), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        const QCtl* bc = reinterpret_cast<const QCtl*>(back.data());
+        uint32_t err[2] = {0, 0};
+        for (int kd = 0; kd < 2; ++kd)
+            for (int x = 0; x < 8; ++x) {
+                const QXcd& X = bc[kd].x[x];
+                err[kd] |= X.error;
+                b->prof[0] += (double)X.t_find * 1e-5;
+                b->prof[1] += (double)X.t_resolve * 1e-5;
+                b->prof[2] += (double)X.t_other * 1e-5;
+                b->prof[3] += (double)X.t_close * 1e-5;
+                b->prof[4] += X.units_resolve;
+                b->prof[5] += X.units_other;
+            }
+        const QDyn* bd = reinterpret_cast<const QDyn*>(back.data() + (o_dyn - o_ctl));
+        for (size_t q = 0; q < n_jobs; ++q)
+            if ((uint32_t)(bd[q].cursor >> 32) != jobs[q].n_phases) err[0] |= 2u;
+        b->rounds += 1;
+        for (size_t q : runs) {
+            tsg_job::Eng& E = b->active[q].job->eng;
+            E.prog.clear();
+            E.args.clear();
+            E.n_at.clear();
+            E.weight = 0;
+        }
+        if (err[0] | err[1]) {
+            // a workgroup saw no progress on its XCD for the idle limit, or a
+            // program did not run to its end: the launch's jobs are not trustworthy
+            for (size_t q = b->active.size(); q-- > 0;)
+                if (std::find(runs.begin(), runs.end(), q) != runs.end())
+                    eng_drop(b, q, fail(TSG_ERR_DEVICE, "the work-queue engine stalled"));
+            return TSG_OK;
+        }
+    }
+    // jobs whose whole program has run
+    for (size_t q = 0; q < b->active.size();) {
+        tsg_job* j = b->active[q].job;
+        if (j->eng.planned_all && j->eng.prog.empty()) {
+            int rc = step_end(j);
+            if (rc) {
+                eng_drop(b, q, rc);
+                continue;
+            }
             b->done.push_back(b->active[q]);
             b->active.erase(b->active.begin() + q);
             continue;
         }
-        RandArgs ra;
-        bool need_rc;
-        if ((rc = step_plan_rc(j, i0, i1, &ra, &need_rc))) return rc;
-        if (need_rc) rands.push_back(ra);
-        if (step_wants_prefetch(j, i0, i1) && (rc = step_prefetch(j, i0))) return rc;
-        units[j->guided ? 1 : 0].push_back({j, i0, i1});
         ++q;
     }
-    const uint32_t nu = (uint32_t)(units[0].size() + units[1].size());
-    if (nu == 0) {
-        // nothing to launch: wait for a pick order instead of spinning
-        if (waiter && (rc = wait_picks(waiter, waiter->step.t))) return rc;
-        return TSG_OK;
-    }
-    // ring slot: wait until the round that last used it has finished
-    const int r = b->ring;
-    b->ring = (b->ring + 1) % tsg_batch::kRing;
-    if (b->ring_used[r]) HIP_TRY(hipEventSynchronize(b->ring_ev[r]));
-    if (nu > b->cap_units) {
-        uint32_t cap = std::max<uint32_t>(nu, 2 * b->cap_units);
-        HIP_TRY(hipDeviceSynchronize());
-        for (int q = 0; q < tsg_batch::kRing; ++q) {
-            auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
-            b->off_rand = up(5 * (size_t)(cap + 2) * sizeof(uint32_t));
-            b->off_units = up(b->off_rand + cap * sizeof(RandArgs));
-            const size_t bytes = b->off_units + cap * sizeof(BatchArgs);
-            if (b->h_blk[q]) HIP_TRY(hipHostFree(b->h_blk[q]));
-            HIP_TRY(hipHostMalloc((void**)&b->h_blk[q], bytes));
-            HIP_TRY(b->d_blk[q].alloc(bytes));
-        }
-        b->cap_units = cap;
-    }
-    BatchArgs* hu = reinterpret_cast<BatchArgs*>(b->h_blk[r] + b->off_units);
-    uint32_t* hi = reinterpret_cast<uint32_t*>(b->h_blk[r]);
-    RandArgs* hr = reinterpret_cast<RandArgs*>(b->h_blk[r] + b->off_rand);
-    const BatchArgs* d_units = reinterpret_cast<const BatchArgs*>(b->d_blk[r].p + b->off_units);
-    const uint32_t* d_idx = reinterpret_cast<const uint32_t*>(b->d_blk[r].p);
-    const RandArgs* d_rand = reinterpret_cast<const RandArgs*>(b->d_blk[r].p + b->off_rand);
-    // layout: kind 0 units then kind 1 units; idx = wg starts (cap+1) then item starts (cap+1), per kind
-    uint32_t off = 0, n_wg[2] = {0, 0}, n_items[2] = {0, 0};
-    uint32_t* wg_s[2] = {hi, hi + (b->cap_units + 1)};
-    uint32_t* it_s[2] = {hi + 2 * (b->cap_units + 1), hi + 3 * (b->cap_units + 1)};
-    // one team size for the round: the units share the device
-    uint64_t round_items = 0;
-    for (int kd = 0; kd < 2; ++kd)
-        for (const U& u : units[kd]) round_items += u.i1 - u.i0;
-    const uint32_t team = team_for(round_items, b->tune.team_max, b->tune.team_waves);
-    const uint32_t ipw = items_per_wg(team);
-    for (int kd = 0; kd < 2; ++kd) {
-        for (size_t q = 0; q < units[kd].size(); ++q) {
-            const U& u = units[kd][q];
-            BatchArgs a = u.j->step.a;
-            a.n_res = (uint32_t)u.j->n_res;
-            a.item0 = (uint32_t)u.i0;
-            a.item1 = (uint32_t)u.i1;
-            a.fused_commit = 0;
-            a.team = team;
-            a.dbg_clk = nullptr;
-            hu[off + q] = a;
-            wg_s[kd][q] = n_wg[kd];
-            it_s[kd][q] = n_items[kd];
-            n_wg[kd] += (uint32_t)((u.i1 - u.i0 + ipw - 1) / ipw);
-            n_items[kd] += (uint32_t)(u.i1 - u.i0);
-        }
-        off += (uint32_t)units[kd].size();
-    }
-    // random candidates of every job that needs a new chunk, one launch
-    uint32_t* rs = hi + 4 * (b->cap_units + 1);
-    uint32_t n_rand = 0;
-    for (size_t q = 0; q < rands.size(); ++q) {
-        hr[q] = rands[q];
-        rs[q] = n_rand;
-        n_rand += rands[q].item1 - rands[q].item0;
-    }
-    HIP_TRY(hipMemcpyAsync(b->d_blk[r].p, b->h_blk[r], b->off_units + nu * sizeof(BatchArgs), hipMemcpyHostToDevice,
-                           s));
-    if (!rands.empty()) {
-        HIP_TRY(launch_rand_multi(d_rand, d_idx + 4 * (b->cap_units + 1), (uint32_t)rands.size(),
-                                  n_rand, s));
-        b->launches += 1;
-    }
-    const BatchArgs* du[2] = {d_units, d_units + units[0].size()};
-    const uint32_t* dw[2] = {d_idx, d_idx + (b->cap_units + 1)};
-    const uint32_t* di[2] = {d_idx + 2 * (b->cap_units + 1), d_idx + 3 * (b->cap_units + 1)};
-    for (int kd = 0; kd < 2; ++kd) {
-        if (units[kd].empty()) continue;
-        HIP_TRY(launch_resolve_multi(du[kd], dw[kd], (uint32_t)units[kd].size(), n_wg[kd], kd == 1, s));
-        HIP_TRY(launch_commit_multi(du[kd], di[kd], (uint32_t)units[kd].size(), n_items[kd], s));
-        b->launches += 2;
-    }
-    HIP_TRY(hipEventRecord(b->ring_ev[r], s));
-    b->ring_used[r] = true;
-    for (int kd = 0; kd < 2; ++kd)
-        for (const U& u : units[kd]) step_advance(u.j, u.i0, u.i1);
-    b->rounds += 1;
-    b->units_total += nu;
     return TSG_OK;
 }
 
@@ -2012,11 +2335,17 @@ int batch_round(tsg_batch* b) {
 int batch_reap(tsg_batch* b, bool wait) {
     for (size_t q = 0; q < b->done.size();) {
         tsg_job* j = b->done[q].job;
-        if (!wait && hipEventQuery(j->ev_run1) == hipErrorNotReady) {
+        if (!wait && j->ev_run1 && hipEventQuery(j->ev_run1) == hipErrorNotReady) {
             ++q;
             continue;
         }
-        b->done[q].status = step_finish(j);
+        if (b->done[q].failed) {
+            if (j->ev_run1) (void)hipEventSynchronize(j->ev_run1);
+            j->step.active = false;
+            j->ran = true;
+        } else {
+            b->done[q].status = step_finish(j);
+        }
         b->taken.push_back(b->done[q]);
         b->done.erase(b->done.begin() + q);
     }
@@ -2286,6 +2615,8 @@ int tsg_job_create(tsg_ctx* ctx, const tsg_params* prm, uint32_t n_examples, con
     HIP_TRY(j->overflow.alloc(1));
     HIP_TRY(j->st_sum.alloc(P + 1));
     HIP_TRY(j->st_cnt.alloc(P + 1));
+    HIP_TRY(j->st_part.alloc((j->N + kStatsChunk - 1) / kStatsChunk));
+    HIP_TRY(j->st_part_n.alloc((j->N + kStatsChunk - 1) / kStatsChunk));
     HIP_TRY(j->done_counter.alloc(1));
     HIP_TRY(j->rank.alloc(j->N));
     HIP_TRY(j->mend.alloc((size_t)j->tiles_x * j->tiles_y));
@@ -2988,6 +3319,8 @@ int tsg_footprint_estimate(const tsg_params* prm, uint32_t n_examples, const uin
     b += maxb * sizeof(uint4);
     b += std::min<uint64_t>(N, std::max<uint64_t>(maxb, kEngineRcBudget / (8 * m))) * m * sizeof(uint2);
     b += (P + 1) * 16 + 512 * 4 + 64;
+    // engine: per-stage cost tables, stage-statistics partial sums
+    b += (P + 1) * 512 * 4 + (N + kStatsChunk - 1) / kStatsChunk * 16;
     // rank map, window mask, prefetched neighbour lists
     const uint64_t win = std::min<uint64_t>(N, kPrefetchCap);
     b += N * 4 + (uint64_t)((out_w + 7) / 8) * ((out_h + 7) / 8) * 12 + win * (8 + 4ull * prm->nearest_neighbors);
@@ -3003,7 +3336,6 @@ int tsg_batch_create(tsg_ctx* ctx, uint32_t max_active, tsg_batch** out) {
     b->ctx = ctx;
     b->max_active = max_active;
     tsg_tuning_default(&b->tune);
-    for (int r = 0; r < tsg_batch::kRing; ++r) HIP_TRY(hipEventCreateWithFlags(&b->ring_ev[r], hipEventDisableTiming));
     *out = b.release();
     return TSG_OK;
 }
@@ -3015,9 +3347,74 @@ int tsg_batch_submit(tsg_batch* b, tsg_job* job, uint64_t* ticket) {
     job->rc_budget = kEngineRcBudget;
     job->in_engine = true;
     job->pk_threads = 2;
+    for (auto* v : {&b->active, &b->pending, &b->done, &b->taken})
+        for (const tsg_batch::Entry& e : *v)
+            if (e.job == job) return fail(TSG_ERR_INVALID_ARGUMENT, "job already submitted");
     *ticket = b->next_ticket++;
-    b->pending.push_back({*ticket, job, 0});
+    b->pending.push_back({*ticket, job, 0, false});
     return TSG_OK;
+}
+
+int tsg_debug_exp_sweep(tsg_ctx* ctx, uint64_t i0, uint64_t n, float* out) {
+    if (!ctx || (!out && n)) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    if (n > (1ull << 28)) return fail(TSG_ERR_INVALID_ARGUMENT, "at most 2^28 values per call");
+    if (n == 0) return TSG_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    DBuf<float> d;
+    HIP_TRY(d.alloc(n));
+    HIP_TRY(launch_exp_sweep(i0, n, d.p, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(out, d.p, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return TSG_OK;
+}
+
+int tsg_debug_exp_sweep_host(uint64_t i0, uint64_t n, float* out) {
+    if (!out && n) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    for (uint64_t r = 0; r < n; ++r) {
+        const uint64_t i = i0 + r;
+        const double t = (double)((i * 0x9E3779B97F4A7C15ull) >> 11) * 0x1p-53;
+        const double x = (i & 3u) ? t * 8.0 : 8.0 + t * 56.0;
+        out[r] = (float)exp_dd(-1.0 * x);
+    }
+    return TSG_OK;
+}
+
+int tsg_batch_engine_profile(tsg_batch* b, double* out6) {
+    if (!b || !out6) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    for (int q = 0; q < 6; ++q) out6[q] = b->prof[q];
+    return TSG_OK;
+}
+
+int tsg_batch_debug_engine(tsg_batch* b, uint32_t wgs_cap, uint32_t flags) {
+    if (!b) return fail(TSG_ERR_INVALID_ARGUMENT, "null batch");
+    b->wgs_cap = wgs_cap;
+    b->engine_flags = flags;
+    return TSG_OK;
+}
+
+int tsg_debug_fail_allocs(int64_t count) {
+    g_fail_allocs.store(count < 0 ? 0 : count);
+    return TSG_OK;
+}
+
+int tsg_batch_cancel(tsg_batch* b, uint64_t ticket) {
+    if (!b) return fail(TSG_ERR_INVALID_ARGUMENT, "null batch");
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    std::lock_guard<std::mutex> lk(b->ctx->mu);
+    for (size_t q = 0; q < b->pending.size(); ++q)
+        if (b->pending[q].ticket == ticket) {
+            tsg_batch::Entry e = b->pending[q];
+            b->pending.erase(b->pending.begin() + q);
+            e.status = TSG_ERR_CANCELLED;
+            e.failed = true;
+            b->taken.push_back(e);
+            return TSG_OK;
+        }
+    // between launches no engine work of an active job is in flight
+    for (size_t q = 0; q < b->active.size(); ++q)
+        if (b->active[q].ticket == ticket) return eng_drop(b, q, TSG_ERR_CANCELLED);
+    return fail(TSG_ERR_INVALID_ARGUMENT, "ticket not pending or running");
 }
 
 int tsg_batch_step(tsg_batch* b, uint32_t max_rounds, uint32_t* in_flight) {

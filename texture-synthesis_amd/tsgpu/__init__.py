@@ -33,7 +33,7 @@ TSG_SAMPLE_ALL, TSG_SAMPLE_IGNORE, TSG_SAMPLE_IMAGE = 0, 1, 2
 STATUS_NAMES = {
     -1: "InvalidRange", -2: "NoExamples", -3: "ExampleGuideMismatch", -4: "SizeMismatch",
     -5: "MapsCountMismatch", -10: "InvalidArgument", -11: "Unsupported", -20: "Device",
-    -21: "OutOfMemory", -22: "Sampling",
+    -21: "OutOfMemory", -22: "Sampling", -23: "Cancelled", -24: "AdmissionRejected",
 }
 
 
@@ -164,6 +164,12 @@ def lib():
     L.tsg_batch_poll.argtypes = [vp, vp, vp, u32, P(u32)]
     L.tsg_batch_take.argtypes = [vp, u64, P(vp)]
     L.tsg_batch_stats.argtypes = [vp, P(u64), P(u64), P(u64)]
+    L.tsg_batch_cancel.argtypes = [vp, u64]
+    L.tsg_debug_fail_allocs.argtypes = [C.c_int64]
+    L.tsg_batch_debug_engine.argtypes = [vp, u32, u32]
+    L.tsg_batch_engine_profile.argtypes = [vp, vp]
+    L.tsg_debug_exp_sweep.argtypes = [vp, u64, u64, vp]
+    L.tsg_debug_exp_sweep_host.argtypes = [u64, u64, vp]
     L.tsg_batch_destroy.argtypes = [vp]
     _lib = L
     return L
@@ -542,10 +548,11 @@ class Session:
 class Batch:
     """Many sessions in flight on one device.
 
-    Each round runs the next Jacobi batch of every active session in one
-    resolve launch and one commit launch; a session's result is identical to
-    Session.run on it alone. No counterpart in the reference, whose
-    Session::run (session.rs:37-66) synthesises one image at a time."""
+    Each round plans every active session as far as its pick orders allow and
+    runs everything planned in one launch of the work-queue engine; a
+    session's result is identical to Session.run on it alone. No counterpart
+    in the reference, whose Session::run (session.rs:37-66) synthesises one
+    image at a time."""
 
     def __init__(self, ctx: Optional[Context] = None, max_active: int = 256):
         self._ctx = ctx or Context.default()
@@ -573,6 +580,22 @@ class Batch:
 
     def run_all(self):
         _check(lib().tsg_batch_run_all(self._h))
+
+    def engine_profile(self) -> dict:
+        """Workgroup milliseconds of the engine launches so far, by activity, and unit counts."""
+        v = np.zeros(6, np.float64)
+        _check(lib().tsg_batch_engine_profile(self._h, _ptr(v)))
+        return dict(find_ms=v[0], resolve_ms=v[1], other_ms=v[2], close_ms=v[3], resolve_units=int(v[4]),
+                    other_units=int(v[5]))
+
+    def debug_engine(self, wgs_cap: int = 0, flags: int = 0):
+        """Test hook: cap the engine's workgroups, set its flags (bit 0: device-scope fences)."""
+        _check(lib().tsg_batch_debug_engine(self._h, int(wgs_cap), int(flags)))
+
+    def cancel(self, ticket: int):
+        """Stops a session that has not finished: poll then reports it with
+        status Cancelled, and take raises it."""
+        _check(lib().tsg_batch_cancel(self._h, int(ticket)))
 
     def poll(self):
         """[(ticket, status)] of finished sessions not yet taken."""
@@ -771,6 +794,26 @@ class SessionBuilder:
 # --------------------------------------------------------------------------
 # single-step entry points (parity tests, tools)
 # --------------------------------------------------------------------------
+def debug_fail_allocs(count: int):
+    """Test hook: the library's next `count` device allocations fail (out of memory)."""
+    _check(lib().tsg_debug_fail_allocs(int(count)))
+
+
+def debug_exp_sweep(i0: int, n: int, ctx: Optional[Context] = None) -> np.ndarray:
+    """Test hook: the device's (float)exp(-x_i) for the sweep arguments (tsg_debug_exp_sweep)."""
+    ctx = ctx or Context.default()
+    out = np.empty(n, np.float32)
+    _check(lib().tsg_debug_exp_sweep(ctx._h, int(i0), int(n), _ptr(out)))
+    return out
+
+
+def debug_exp_sweep_host(i0: int, n: int) -> np.ndarray:
+    """Test hook: the library's weight exp on the host for the sweep arguments."""
+    out = np.empty(n, np.float32)
+    _check(lib().tsg_debug_exp_sweep_host(int(i0), int(n), _ptr(out)))
+    return out
+
+
 def pyramid(img: np.ndarray, levels: int, ctx: Optional[Context] = None) -> np.ndarray:
     ctx = ctx or Context.default()
     I = _Images([img])

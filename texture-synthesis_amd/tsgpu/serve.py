@@ -4,9 +4,12 @@ Requests (unbuilt SessionBuilders) wait in arrival order. A request is
 admitted when the device bytes of the admitted requests plus its own
 estimate fit the memory budget and an engine slot is free: it is then built
 (pyramids and state go to the device and stay there) and handed to a
-``Batch``, which advances every admitted request by one Jacobi batch per
-round. Requests that arrive while others run join at the next round, and a
-finished request frees its memory and slot for the next in line.
+``Batch``, whose rounds run every admitted request's planned work in one
+launch of the work-queue engine. Requests that arrive while others run join
+at the next round, and a finished request frees its memory and slot for the
+next in line. A request whose estimate alone exceeds the budget is rejected
+(AdmissionRejected) rather than held; a request can be cancelled; a request
+that fails (invalid, out of memory) is reported without disturbing others.
 
 Each result is the same as running the request alone (``Session.run``):
 the engine only interleaves launches, it never changes a request's
@@ -20,7 +23,10 @@ from collections import deque
 from dataclasses import dataclass, field
 from typing import Callable, Deque, Dict, List, Optional
 
-from . import Batch, Context, GeneratedImage, SessionBuilder
+from . import Batch, Context, GeneratedImage, SessionBuilder, TsgError
+
+ADMISSION_REJECTED = -24
+CANCELLED = -23
 
 
 @dataclass
@@ -80,6 +86,8 @@ class Server:
         self.in_use = 0
         self.peak_in_use = 0
         self.memory_holds = 0  # admissions deferred because the budget was full
+        self.rejected = 0      # requests larger than the whole budget
+        self._cancelled: List[Finished] = []
         self._next = 0
 
     def submit(self, builder: SessionBuilder, arrival: Optional[float] = None, schedule: Optional[dict] = None) -> int:
@@ -91,10 +99,36 @@ class Server:
                                      builder.footprint_estimate(), out.width * out.height, schedule))
         return rid
 
+    def cancel(self, rid: int) -> bool:
+        """Cancels a request that has not finished: a waiting one is dropped,
+        a running one stops at the engine's next launch. It is reported by
+        ``pump`` with a Cancelled error. False when rid is unknown or done."""
+        for w in list(self.waiting):
+            if w.rid == rid:
+                self.waiting.remove(w)
+                t = self.clock()
+                self._cancelled.append(Finished(rid, None, TsgError(CANCELLED, "request cancelled while waiting"),
+                                                w.arrival, t, t, 0, w.out_px))
+                return True
+        for ticket, r in self.running.items():
+            if r.rid == rid:
+                self.batch.cancel(ticket)
+                return True
+        return False
+
     def _admit(self) -> List[Finished]:
-        failed = []
+        failed, self._cancelled = self._cancelled, []
         while self.waiting and len(self.running) < self.max_active:
             w = self.waiting[0]
+            if w.estimate > self.hbm_budget:
+                # can never fit, whatever else finishes: rejected, not held
+                self.waiting.popleft()
+                t = self.clock()
+                err = TsgError(ADMISSION_REJECTED, f"request needs {w.estimate} device bytes, the budget is "
+                                                   f"{self.hbm_budget}")
+                self.rejected += 1
+                failed.append(Finished(w.rid, None, err, w.arrival, t, t, 0, w.out_px))
+                continue
             if self.running and self.in_use + w.estimate > self.hbm_budget:
                 self.memory_holds += 1
                 break
@@ -132,7 +166,7 @@ class Server:
         return done
 
     def idle(self) -> bool:
-        return not self.waiting and not self.running
+        return not self.waiting and not self.running and not self._cancelled
 
     def drain(self) -> List[Finished]:
         out = []
