@@ -302,14 +302,15 @@ int tsg_resize(tsg_ctx* ctx, const tsg_image* img, uint32_t nw, uint32_t nh, int
 /* ---- test and tooling entry points (no counterpart in the reference API;
  *      they expose single steps of the path for parity tests) ---- */
 
-/* Where the engine's workgroup time went, summed over the batch's launches
- * and workgroups: milliseconds searching for units (idle waits included),
- * in resolve units, in other units, closing phases; then the counts of
- * resolve units and other units. */
-int tsg_batch_engine_profile(tsg_batch* b, double* out6);
+/* Where the engine's time went, summed over the batch's launches: workgroup
+ * milliseconds searching for units (idle waits included), in resolve units,
+ * in other units, closing phases; the counts of resolve units and other
+ * units; then host milliseconds admitting jobs, planning their programs,
+ * packing and queueing the launch block, from the launches to their
+ * completion, and finishing jobs (11 values). */
+int tsg_batch_engine_profile(tsg_batch* b, double* out11);
 /* Test hook of the work-queue engine: at most wgs_cap workgroups per launch
- * (0: as many as are resident) and engine flags (bit 0: device-scope
- * acquire and release around every work unit). */
+ * (0: as many as are resident); flags are reserved (0). */
 int tsg_batch_debug_engine(tsg_batch* b, uint32_t wgs_cap, uint32_t flags);
 
 /* Failure injection for tests of error handling: the library's next `count`

@@ -255,7 +255,7 @@ constexpr uint32_t kQElems = 512;
 
 struct QPhase {
     uint32_t kind;
-    uint32_t stage;    // slot of the job's stage arguments
+    uint32_t stage;    // stage of the job (keys the cost tables cached in LDS)
     uint32_t i0, i1;
     uint32_t n_res;    // resolve: resolved pixels at the batch start
     uint32_t team;     // resolve: waves per item
@@ -265,7 +265,9 @@ struct QPhase {
                                  // for a new-pixel window (mask_new, tile_min, rank)
     uint32_t units;
     uint32_t fuse;     // resolve: the closing workgroup commits the batch (no commit phase)
-    uint32_t arg;      // index of the phase's batch arguments in the launch's argument array
+    uint32_t arg;      // index of the stage's batch arguments in the launch's argument array; a
+                       // resolve phase's item range, n_res, team, rc_base and prefetch window
+                       // (with the job's neighbour-list buffers) replace the stage's
 };
 
 struct QJob {

@@ -1110,7 +1110,7 @@ __device__ __forceinline__ void flow_wait(const FlowArgs& f, const BatchArgs& a,
 
 // Returns the result in the leader's lane 0 (zero elsewhere).
 template <bool GUIDED>
-__device__ uint4 resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t item, uint32_t item0, uint32_t n_res,
+__device__ __forceinline__ uint4 resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t item, uint32_t item0, uint32_t n_res,
                               WaveLds* Lt, int member, int T, bool valid, const float* lut, const float* glut,
                               int lane, const FlowArgs* flow = nullptr) {
     uint4 res = make_uint4(0, 0, 0, 0);
@@ -2007,6 +2007,23 @@ __device__ __forceinline__ uint32_t xcc_id() {
 // dropped).
 __device__ __forceinline__ void unit_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 
+// A copy of a record that every lane reads at the same address (job
+// programs, phases, argument blocks), each 32-bit word moved to a scalar
+// register as it arrives. After an acquire fence the compiler cannot treat
+// such records as unchanged since the launch, so it reads them with vector
+// loads; held as loaded, their fields took some 30 VGPRs plus spills through
+// the whole resolve loop, while as scalars they cost no VGPRs.
+template <class T>
+__device__ __forceinline__ T ld_uniform(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "record of whole words");
+    T v;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(p);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&v);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = (uint32_t)__builtin_amdgcn_readfirstlane((int)src[i]);
+    return v;
+}
+
 // Release to the workgroups of the same XCD: the wave waits until its stores
 // have completed to the XCD's L2, which every CU of the XCD reads through.
 // A job's workgroups all run on its XCD, so the L2 write-back of a
@@ -2166,11 +2183,21 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) __attribute__((amdgpu_waves_per
         const uint32_t jb = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_job);
         const uint32_t phi = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_ph);
         const uint32_t unit = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_unit);
-        const QJob& J = e.jobs[jb];
-        const QPhase P = J.phases[phi];
+        const QJob J = ld_uniform(e.jobs + jb);
+        const QPhase P = ld_uniform(J.phases + phi);
         const uint32_t tid = threadIdx.x;
         if (P.kind == Q_RESOLVE) {
-            const BatchArgs& a = args[P.arg];
+            BatchArgs a = ld_uniform(args + P.arg);
+            a.item0 = P.i0;
+            a.item1 = P.i1;
+            a.n_res = P.n_res;
+            a.team = P.team;
+            a.rc_base = P.rc_base;
+            const bool pre = P.pre_end > P.pre_base;
+            a.pre_nbrs = pre ? J.pre_nbrs : nullptr;
+            a.pre_cnt = pre ? J.pre_cnt : nullptr;
+            a.pre_base = P.pre_base;
+            a.pre_end = P.pre_end;
             // the job stage's cost tables and map descriptors in LDS
             const uint32_t key = ((jb << 4) | P.stage) + 1u;
             const bool reload = s_key != key;
@@ -2192,18 +2219,18 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) __attribute__((amdgpu_waves_per
                                  glut, ln);
             if (reload && tid == 0) s_key = key;
         } else if (P.kind == Q_COMMIT) {
-            const BatchArgs& a = args[P.arg];
+            const BatchArgs a = ld_uniform(args + P.arg);
             const uint32_t i = P.i0 + unit * kQElems + tid;
             if (tid < kQElems && i < P.i1)
                 commit_one(a.W, a.tiles_x, a.maps, a.work, J.result, i, P.i0, a.first_new, a.state_w, a.mask_w,
                            a.score_w, a.gmask);
         } else if (P.kind == Q_RAND) {
-            const BatchArgs& a = args[P.arg];
+            const BatchArgs a = ld_uniform(args + P.arg);
             const RandArgs ra{a.maps, a.n_maps, a.stage_seed, P.i0, P.i1, a.m, J.rc, J.overflow};
             const uint32_t i = P.i0 + unit * kQElems + tid;
             if (tid < kQElems && i < P.i1) rand_item(ra, i);
         } else if (P.kind == Q_RECOLOR) {
-            const BatchArgs& a = args[P.arg];
+            const BatchArgs a = ld_uniform(args + P.arg);
             const uint32_t i = P.i0 + unit * kQElems + tid;
             if (tid < kQElems && i < P.i1) {
                 const uint32_t p = a.rlist[i];
@@ -2212,7 +2239,7 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) __attribute__((amdgpu_waves_per
                 reinterpret_cast<uint32_t*>(&a.state_w[p])[0] = md.px[(st.y >> 16) * md.w + (st.y & 0xffffu)];
             }
         } else if (P.kind == Q_STATS) {
-            const BatchArgs& a = args[P.arg];
+            const BatchArgs a = ld_uniform(args + P.arg);
             const uint32_t lo = P.i0 + unit * kStatsChunk;
             const uint32_t hi = min(P.i1, lo + kStatsChunk);
             double* red = reinterpret_cast<double*>(wls);
@@ -2231,7 +2258,7 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) __attribute__((amdgpu_waves_per
                 J.tmin[t] = 0xffffffffu;
             }
         } else if (P.kind == Q_MASKOR) {
-            const BatchArgs& a = args[P.arg];
+            const BatchArgs a = ld_uniform(args + P.arg);
             const uint32_t q = P.i0 + unit * kQElems + tid;
             if (tid < kQElems && q < P.i1) {
                 const uint32_t p = a.rlist[q], px = p % a.W, py = p / a.W;
@@ -2240,7 +2267,7 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) __attribute__((amdgpu_waves_per
                 atomicMin(&J.tmin[t], q);
             }
         } else if (P.kind == Q_PREFETCH) {
-            const BatchArgs& a = args[P.arg];
+            const BatchArgs a = ld_uniform(args + P.arg);
             PrefetchArgs pa{};
             pa.W = a.W;
             pa.H = a.H;
@@ -2283,7 +2310,7 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) __attribute__((amdgpu_waves_per
             const uint64_t c3 = tid == 0 ? wall_clock64() : 0;
             // closing work of the phase: every unit's results are in the L2
             if (P.kind == Q_RESOLVE && P.fuse) {
-                const BatchArgs& a = args[P.arg];
+                const BatchArgs a = ld_uniform(args + P.arg);
                 for (uint32_t i = P.i0 + tid; i < P.i1; i += blockDim.x)
                     commit_one(a.W, a.tiles_x, a.maps, a.work, J.result, i, P.i0, a.first_new, a.state_w, a.mask_w,
                                a.score_w, a.gmask);

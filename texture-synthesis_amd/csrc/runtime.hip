@@ -1871,7 +1871,7 @@ struct tsg_batch {
     uint64_t rounds = 0, launches = 0, units_total = 0;
     uint64_t idle_ticks = 200000000ull;  // 2 s of the 100 MHz clock without progress on an XCD
     uint32_t wgs_cap = 0;                 // test hook: at most this many engine workgroups (0: all resident)
-    double prof[16] = {};                 // accumulated QXcd clocks and counts (tsg_batch_engine_profile)
+    double prof[16] = {};                 // accumulated QXcd clocks and counts, host phase times (tsg_batch_engine_profile)
     uint32_t engine_flags = 0;            // QEngineArgs::flags
     tsg_tuning tune{};
     ~tsg_batch() {
@@ -2044,30 +2044,21 @@ int eng_plan(tsg_job* j) {
         // neighbour lists ahead of latency-bound batches shorten a lone job's
         // chain; with other jobs to run meanwhile the resolve searches itself
         if (j->eng.sharing <= kEnginePrefetchJobs && step_wants_prefetch(j, i0, i1)) eng_prefetch(j, i0);
-        BatchArgs a = st.a;
-        a.n_res = (uint32_t)j->n_res;
-        a.item0 = (uint32_t)i0;
-        a.item1 = (uint32_t)i1;
-        a.rc_base = (uint32_t)st.rc_lo;
         // teams of waves per item shorten a batch's latency at the cost of
         // throughput; the jobs of a launch share the device, so the team is
         // sized against the waves per job
-        a.team = team_for((i1 - i0) * j->eng.sharing, j->tune.team_max, j->tune.team_waves);
-        if (!(a.pre_nbrs && i0 >= a.pre_base && i1 <= a.pre_end)) {
-            a.pre_nbrs = nullptr;
-            a.pre_cnt = nullptr;
-            a.pre_base = a.pre_end = 0;
-        }
-        const uint32_t arg = (uint32_t)j->eng.args.size();
-        j->eng.args.push_back(a);
+        const uint32_t team = team_for((i1 - i0) * j->eng.sharing, j->tune.team_max, j->tune.team_waves);
         const uint32_t n = (uint32_t)(i1 - i0);
-        eng_push(j, Q_RESOLVE, (uint32_t)i0, (uint32_t)i1, items_per_wg(a.team), arg);
+        // the batch's own arguments ride on its phase; the rest are the stage's
+        eng_push(j, Q_RESOLVE, (uint32_t)i0, (uint32_t)i1, items_per_wg(team), kNoArg);
         QPhase& q = j->eng.prog.back();
-        q.n_res = a.n_res;
-        q.team = a.team;
-        q.rc_base = a.rc_base;
-        q.pre_base = a.pre_base;
-        q.pre_end = a.pre_end;
+        q.n_res = (uint32_t)j->n_res;
+        q.team = team;
+        q.rc_base = (uint32_t)st.rc_lo;
+        if (st.a.pre_nbrs && i0 >= st.a.pre_base && i1 <= st.a.pre_end) {
+            q.pre_base = st.a.pre_base;
+            q.pre_end = st.a.pre_end;
+        }
         q.fuse = n <= kEngineFuseMax;
         if (!q.fuse) eng_push(j, Q_COMMIT, (uint32_t)i0, (uint32_t)i1, kQElems, kNoArg);
         j->eng.weight += (uint64_t)n * (j->prm.nearest_neighbors + 8);
@@ -2134,9 +2125,20 @@ size_t up256(size_t v) { return (v + 255) & ~(size_t)255; }
 // XCDs by work (largest first onto the least loaded), their programs and
 // batch arguments go to the device in one copy, then the unguided and the
 // guided engine kernels run.
+double host_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 int batch_round(tsg_batch* b) {
     hipStream_t s = b->ctx->stream;
+    double t_mark = host_ms();
+    auto lap = [&](int slot) {
+        const double t = host_ms();
+        b->prof[slot] += t - t_mark;
+        t_mark = t;
+    };
     batch_admit(b);
+    lap(6);
     if (b->active.empty()) return TSG_OK;
     // plan every job as far as its picks allow; a job that cannot be planned leaves
     for (size_t q = 0; q < b->active.size();) {
@@ -2151,6 +2153,7 @@ int batch_round(tsg_batch* b) {
     std::vector<size_t> runs;
     for (size_t q = 0; q < b->active.size(); ++q)
         if (!b->active[q].job->eng.prog.empty()) runs.push_back(q);
+    lap(7);
     if (runs.empty()) {
         // nothing planned: wait for one job's next pick order
         for (const tsg_batch::Entry& e : b->active)
@@ -2261,6 +2264,7 @@ int batch_round(tsg_batch* b) {
             for (uint32_t y = 0; y <= 8; ++y) C.job0[y] += (uint32_t)first[kd];
         }
         HIP_TRY(hipMemcpyAsync(D, H, o_nat + n_at * 4, hipMemcpyHostToDevice, s));
+        lap(8);
         for (int kd = 0; kd < 2; ++kd) {
             if (!count[kd]) continue;
             if (!b->wgs[kd]) return fail(TSG_ERR_DEVICE, "engine occupancy query failed");
@@ -2280,6 +2284,7 @@ int batch_round(tsg_batch* b) {
         std::vector<char> back(o_ph - o_ctl);
         HIP_TRY(hipMemcpyAsync(back.data(), D + o_ctl, back.size(), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
+        lap(9);
         const QCtl* bc = reinterpret_cast<const QCtl*>(back.data());
         uint32_t err[2] = {0, 0};
         for (int kd = 0; kd < 2; ++kd)
@@ -2328,6 +2333,7 @@ int batch_round(tsg_batch* b) {
         }
         ++q;
     }
+    lap(10);
     return TSG_OK;
 }
 
@@ -3380,9 +3386,9 @@ int tsg_debug_exp_sweep_host(uint64_t i0, uint64_t n, float* out) {
     return TSG_OK;
 }
 
-int tsg_batch_engine_profile(tsg_batch* b, double* out6) {
-    if (!b || !out6) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
-    for (int q = 0; q < 6; ++q) out6[q] = b->prof[q];
+int tsg_batch_engine_profile(tsg_batch* b, double* out11) {
+    if (!b || !out11) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
+    for (int q = 0; q < 11; ++q) out11[q] = b->prof[q];
     return TSG_OK;
 }
 

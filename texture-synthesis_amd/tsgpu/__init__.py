@@ -582,14 +582,16 @@ class Batch:
         _check(lib().tsg_batch_run_all(self._h))
 
     def engine_profile(self) -> dict:
-        """Workgroup milliseconds of the engine launches so far, by activity, and unit counts."""
-        v = np.zeros(6, np.float64)
+        """Workgroup milliseconds of the engine launches so far, by activity, the
+        unit counts, and host milliseconds by step of the rounds."""
+        v = np.zeros(11, np.float64)
         _check(lib().tsg_batch_engine_profile(self._h, _ptr(v)))
         return dict(find_ms=v[0], resolve_ms=v[1], other_ms=v[2], close_ms=v[3], resolve_units=int(v[4]),
-                    other_units=int(v[5]))
+                    other_units=int(v[5]), host_admit_ms=v[6], host_plan_ms=v[7], host_pack_ms=v[8],
+                    launch_to_done_ms=v[9], host_finish_ms=v[10])
 
     def debug_engine(self, wgs_cap: int = 0, flags: int = 0):
-        """Test hook: cap the engine's workgroups, set its flags (bit 0: device-scope fences)."""
+        """Test hook: cap the engine's workgroups; flags are reserved (0)."""
         _check(lib().tsg_batch_debug_engine(self._h, int(wgs_cap), int(flags)))
 
     def cancel(self, ticket: int):
