@@ -265,6 +265,7 @@ struct QPhase {
                                  // for a new-pixel window (mask_new, tile_min, rank)
     uint32_t units;
     uint32_t fuse;     // resolve: the closing workgroup commits the batch (no commit phase)
+    uint32_t rep;      // resolve: items each team takes in turn within one unit
     uint32_t arg;      // index of the stage's batch arguments in the launch's argument array; a
                        // resolve phase's item range, n_res, team, rc_base and prefetch window
                        // (with the job's neighbour-list buffers) replace the stage's
