@@ -124,8 +124,9 @@ def batched_specs(n, seed=4000):
     return [(int(ex[j]), int(out[j]), int(k[j]), seed + j) for j in range(n)]
 
 
-def batched_leg(ts, synth, n):
-    """n syntheses one after another (Session.run) and all at once (Batch)."""
+def batched_leg(ts, synth, n, sequential=True):
+    """n syntheses one after another (Session.run) and all at once (Batch);
+    sequential=False skips the one-after-another runs (profiling the engine)."""
     specs = batched_specs(n)
     exs = [synth.example(sd, e, e) for e, _, _, sd in specs]
 
@@ -136,16 +137,18 @@ def batched_leg(ts, synth, n):
     px = sum(o * o for _, o, _, _ in specs)
     # timed, as the plain line: the syntheses with their outputs left in
     # device memory (the downloads are timed apart); the digests after
-    sess = [b.build() for b in builders()]
-    t0 = time.perf_counter()
-    for s in sess:
-        s.run_resident()
-    t_seq = time.perf_counter() - t0
-    seq = []
-    for s in sess:
-        seq.append(digest(s.download()))
-        s.close()
-    del sess
+    t_seq, seq = None, None
+    if sequential:
+        sess = [b.build() for b in builders()]
+        t0 = time.perf_counter()
+        for s in sess:
+            s.run_resident()
+        t_seq = time.perf_counter() - t0
+        seq = []
+        for s in sess:
+            seq.append(digest(s.download()))
+            s.close()
+        del sess
     sess = [b.build() for b in builders()]
     batch = ts.Batch(ts.Context.default(), max_active=n)
     t0 = time.perf_counter()
@@ -164,12 +167,12 @@ def batched_leg(ts, synth, n):
                     f"{{128,256,512,1024}}^2, k in {{20,50}}, m=50, seeds 4000+j",
         "output_mpix": round(px / 1e6, 3),
         "value": round(px / t_bat / 1e6, 3), "unit": "Mpix/s",
-        "sequential_value": round(px / t_seq / 1e6, 3),
-        "speedup_vs_sequential": round(t_seq / t_bat, 2),
-        "seconds": round(t_bat, 3), "sequential_seconds": round(t_seq, 3),
+        "sequential_value": round(px / t_seq / 1e6, 3) if t_seq else None,
+        "speedup_vs_sequential": round(t_seq / t_bat, 2) if t_seq else None,
+        "seconds": round(t_bat, 3), "sequential_seconds": round(t_seq, 3) if t_seq else None,
         "pcie_inclusive": {"value": round(px / t_pcie / 1e6, 3), "unit": "Mpix/s", "seconds": round(t_pcie, 3),
                            "note": "the batched syntheses with every result downloaded to host memory"},
-        "identical_to_alone": seq == got,
+        "identical_to_alone": (seq == got) if seq is not None else None,
         "launches": st["launches"], "engine_rounds": st["rounds"],
         "engine_workgroup_ms": {k: (round(v, 1) if isinstance(v, float) else v) for k, v in prof.items()},
     }
@@ -224,14 +227,20 @@ def serving_specs(n, seed=5000):
     return [(seed + i, int(sides[i]), bool(inpaint[i])) for i in range(n)]
 
 
-def serving_leg(ts, synth, n, qps, budget_gib):
+def serving_leg(ts, synth, n, qps, budget_gib, quantum=None, rounds_per_pump=1):
     from tsgpu import serve
     import numpy as np
     reqs = [(sp, sp[1] * sp[1], sp[2]) for sp in serving_specs(n)]
+    # the requests' images are generated before the clock starts (a client
+    # sends finished images; the generator is not part of the server), on
+    # host threads (numpy releases the interpreter lock in its array work)
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max(1, min(16, host_cpu()[1]))) as pool:
+        built = list(pool.map(lambda r: serving_request(ts, synth, *r[0]), reqs))
     gaps = np.random.default_rng(5999).exponential(1.0 / qps, n)
     arrivals = np.cumsum(gaps) - gaps[0]
     srv = serve.Server(ts.Context.default(), hbm_budget=int(budget_gib * (1 << 30)), max_active=256,
-                       rounds_per_pump=4)
+                       rounds_per_pump=rounds_per_pump, quantum=serve.QUANTUM if quantum is None else quantum)
     fin = []
     busy = 0.0  # wall time with at least one request admitted or waiting
     t0 = time.perf_counter()
@@ -239,7 +248,8 @@ def serving_leg(ts, synth, n, qps, budget_gib):
     while i < n or not srv.idle():
         now = time.perf_counter() - t0
         while i < n and arrivals[i] <= now:
-            srv.submit(serving_request(ts, synth, *reqs[i][0]), arrival=t0 + arrivals[i])
+            srv.submit(built[i], arrival=t0 + arrivals[i])
+            built[i] = None
             i += 1
         if srv.idle():
             time.sleep(max(0.0, min(0.01, arrivals[i] - now)))
@@ -248,6 +258,7 @@ def serving_leg(ts, synth, n, qps, budget_gib):
         fin.extend(srv.pump())
         busy += time.perf_counter() - ta
     t_end = time.perf_counter()
+    launches = srv.batch.stats()["launches"]
     srv.close()
     errors = [f for f in fin if f.error is not None]
     lat = [f.latency_s * 1e3 for f in fin if f.error is None]
@@ -263,6 +274,7 @@ def serving_leg(ts, synth, n, qps, budget_gib):
         "busy_fraction": round(busy / (t_end - t0), 3),
         "hbm_budget_gib": budget_gib, "peak_admitted_gib": round(srv.peak_in_use / (1 << 30), 3),
         "memory_holds": srv.memory_holds, "rejected": srv.rejected,
+        "engine": {"quantum_items": srv.quantum, "rounds_per_pump": rounds_per_pump, "launches": launches},
     }
 
 
@@ -307,6 +319,8 @@ def main():
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="time the library's default launch structure")
     ap.add_argument("--batched", type=int, default=256, help="syntheses in the batched leg (0 = skip)")
+    ap.add_argument("--no-sequential", dest="sequential", action="store_false",
+                    help="batched leg without the one-after-another runs (for profiling the engine)")
     ap.add_argument("--guided", type=int, default=3, help="runs of the guided leg (0 = skip)")
     ap.add_argument("--serve-requests", type=int, default=2000, help="requests in the serving leg (0 = skip)")
     ap.add_argument("--serve-tight-requests", type=int, default=300,
@@ -410,7 +424,7 @@ def main():
     if args.guided:
         out["guided"] = guided_leg(ts, synth, args.guided)
     if args.batched:
-        out["batched"] = batched_leg(ts, synth, args.batched)
+        out["batched"] = batched_leg(ts, synth, args.batched, args.sequential)
         out["batched"]["vs_plain"] = round(out["batched"]["value"] / value, 2)
     if args.serve_requests:
         out["serving"] = serving_leg(ts, synth, args.serve_requests, args.serve_qps, args.serve_budget_gib)

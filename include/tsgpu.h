@@ -249,13 +249,19 @@ int tsg_job_phase_clocks(tsg_job* job, uint64_t* out, uint64_t n_items);
  * are their execution spans on the device clock (earliest workgroup start
  * to latest workgroup end); the others' come from events around them. */
 int tsg_job_kernel_stats(tsg_job* job, double* ms, uint64_t* items, uint64_t* launches);
+/* Frees the job. A job submitted to a tsg_batch belongs to the batch until
+ * tsg_batch_take hands it back: destroying it before is refused (the job
+ * stays, tsg_last_error says why) and the batch frees it with itself. */
 void tsg_job_destroy(tsg_job* job);
 
 /* ---- many syntheses at once (no counterpart in the reference: serving and
  *      batch callers that run many Session::run at the same time) ----
- * A tsg_batch advances every admitted job in lockstep rounds: each round
- * resolves every active job's next Jacobi batch in one launch and commits
- * them in a second. A job's result is identical to tsg_job_run on it alone.
+ * A tsg_batch runs its admitted jobs together. Each step (a "round") plans
+ * every job's next phases exactly as a run alone would batch them (random
+ * candidates, each Jacobi batch's resolve and commit, stage statistics) and
+ * runs all of them in one launch of a persistent work-queue kernel, whose
+ * workgroups claim the jobs' work units as they free up. A job's result is
+ * identical to tsg_job_run on it alone.
  * Jobs are created with tsg_job_create and handed over with
  * tsg_batch_submit; once finished they are taken back with tsg_batch_take
  * (then downloaded and destroyed by the caller). */
@@ -272,6 +278,22 @@ int tsg_footprint_estimate(const tsg_params* p, uint32_t n_examples, const uint3
 /* At most max_active jobs run at a time; the rest wait in submission order. */
 int tsg_batch_create(tsg_ctx* ctx, uint32_t max_active, tsg_batch** out);
 int tsg_batch_submit(tsg_batch* b, tsg_job* job, uint64_t* ticket);
+/* Resolve items each job may plan per round (0, the default: everything its
+ * pick orders allow). A bound keeps rounds short, so that jobs submitted
+ * while others run start soon (serving); a job still plans at least one
+ * Jacobi batch per round. Results do not depend on it. */
+int tsg_batch_set_quantum(tsg_batch* b, uint64_t items);
+/* A job that starts while fewer than `jobs` jobs are in the batch (running
+ * and waiting, itself included) runs alone instead: to its end, on the
+ * single-job path with the whole device (tsg_job_run's launch sequence),
+ * before the round's engine launch; a round's alone jobs run smallest first.
+ * Its stage-ahead buffers, beyond tsg_footprint_estimate, are used only when
+ * they fit ahead_bytes, and are released when its run ends. jobs = 0, the
+ * default: every job runs on the engine. Under light load a lone job's chain
+ * of small batches is what bounds its latency, and the single-job path
+ * shortens that chain; under load the engine's throughput wins. Results do
+ * not depend on it. */
+int tsg_batch_set_alone_below(tsg_batch* b, uint32_t jobs, uint64_t ahead_bytes);
 /* Runs up to max_rounds rounds; in_flight = jobs not yet reported finished. */
 int tsg_batch_step(tsg_batch* b, uint32_t max_rounds, uint32_t* in_flight);
 /* Runs until every submitted job has finished. */
@@ -310,7 +332,8 @@ int tsg_resize(tsg_ctx* ctx, const tsg_image* img, uint32_t nw, uint32_t nh, int
  * completion, and finishing jobs (11 values). */
 int tsg_batch_engine_profile(tsg_batch* b, double* out11);
 /* Test hook of the work-queue engine: at most wgs_cap workgroups per launch
- * (0: as many as are resident); flags are reserved (0). */
+ * (0: as many as are resident); flags bit 0 spreads every launch over all
+ * XCDs (by default only launches of fewer than 8 jobs are). */
 int tsg_batch_debug_engine(tsg_batch* b, uint32_t wgs_cap, uint32_t flags);
 
 /* Failure injection for tests of error handling: the library's next `count`

@@ -69,11 +69,15 @@ def assert_same_result(a, b, what):
         assert np.array_equal(x, y), what
 
 
-@pytest.mark.parametrize("max_active", [64, 3])
-def test_batch_equals_alone(ctx, max_active):
+@pytest.mark.parametrize("max_active,spread", [(64, False), (3, False), (64, True)])
+def test_batch_equals_alone(ctx, max_active, spread):
+    # 64: the unguided jobs' launches home jobs on XCDs; 3 (fewer jobs than
+    # XCDs) and the forced flag: spread launches, every XCD on every job
     specs = spec_list()
     alone = [build(s).run() for s in specs]
     batch = ts.Batch(ctx, max_active=max_active)
+    if spread:
+        batch.debug_engine(0, 1)
     tickets = [batch.submit(build(s)) for s in specs]
     batch.run_all()
     finished = dict(batch.poll())
@@ -82,6 +86,45 @@ def test_batch_equals_alone(ctx, max_active):
         assert_same_result(batch.take(t), a, f"{s[0]}")
     st = batch.stats()
     assert st["units"] > 0 and st["rounds"] > 0
+    batch.close()
+
+
+@pytest.mark.parametrize("quantum,max_active", [(1, 64), (300, 4), (5000, 64)])
+def test_batch_quantum_equals_alone(ctx, quantum, max_active):
+    # a planning quantum cuts a job's program at Jacobi batch boundaries, so
+    # its stages span several launches (quantum 1: one batch per launch);
+    # with at most 8 jobs at a time the launches also plan neighbour lists ahead
+    specs = spec_list()
+    alone = [build(s).run() for s in specs]
+    batch = ts.Batch(ctx, max_active=max_active, quantum=quantum)
+    tickets = [batch.submit(build(s)) for s in specs]
+    batch.run_all()
+    finished = dict(batch.poll())
+    assert sorted(finished) == sorted(tickets) and not any(finished.values())
+    for t, s, a in zip(tickets, specs, alone):
+        assert_same_result(batch.take(t), a, f"{s[0]} quantum {quantum}")
+    # the stages really were cut: more launches than there are stages
+    assert batch.stats()["launches"] > 12
+    batch.close()
+
+
+@pytest.mark.parametrize("alone_below,max_active,ahead", [(100, 64, 0), (100, 64, 1 << 34), (6, 3, 1 << 34)])
+def test_batch_alone_lane_equals_alone(ctx, alone_below, max_active, ahead):
+    # jobs that start while few are in the batch run alone on the single-job
+    # path (100: all of them; 6 with 3 slots: the first ones share the
+    # engine, the last ones run alone), with or without stage-ahead buffers
+    specs = spec_list()
+    alone = [build(s).run() for s in specs]
+    batch = ts.Batch(ctx, max_active=max_active)
+    batch.set_alone(alone_below, ahead)
+    tickets = [batch.submit(build(s)) for s in specs]
+    batch.run_all()
+    finished = dict(batch.poll())
+    assert sorted(finished) == sorted(tickets) and not any(finished.values())
+    for t, s, a in zip(tickets, specs, alone):
+        assert_same_result(batch.take(t), a, f"{s[0]} alone_below {alone_below}")
+    launches = batch.stats()["launches"]
+    assert (launches == 0) if alone_below > len(specs) else (launches > 0)
     batch.close()
 
 

@@ -330,13 +330,20 @@ static_assert(sizeof(QCtl) == 256 * 9, "job table offsets, then one block per XC
 // The launch's batch arguments: every resolve phase has its own entry (its
 // batch, candidate chunk and neighbour-list window filled in), every other
 // phase shares its stage's entry.
+// A spread launch: every workgroup, on any XCD, may take units of any of its
+// jobs (all listed as on XCD 0, whose control words every workgroup uses),
+// and a unit's results are released to device scope (the L2 written back).
+// For launches with fewer jobs than XCDs, which home placement would leave
+// on a few XCDs.
+constexpr uint32_t kEngineSpread = 1u;
+
 struct QEngineArgs {
     const QJob* jobs;
     QDyn* dyn;
     QCtl* ctl;
     const int32_t* offtab;
     uint32_t offtab_n;
-    uint32_t flags;          // reserved (0)
+    uint32_t flags;          // kEngineSpread: the launch's jobs are shared by every XCD
     uint64_t idle_ticks;
 };
 
@@ -348,10 +355,6 @@ hipError_t launch_exp_sweep(uint64_t i0, uint64_t n, float* out, hipStream_t s);
 // XCD of each of n workgroups of a probe launch (test hook: the engine's
 // job placement relies on the hardware XCC id).
 hipError_t launch_xcc_probe(uint32_t* out, uint32_t n, hipStream_t s);
-// random candidates of several jobs: unit u's items start at item_start[u]
-// of the launch's `total` threads (item_start[0] = 0)
-hipError_t launch_rand_multi(const RandArgs* units, const uint32_t* item_start, uint32_t n_units, uint32_t total,
-                             hipStream_t s);
 // rank[rlist[q]] = q for q in [lo, hi)
 hipError_t launch_rank(const uint32_t* rlist, uint32_t lo, uint32_t hi, uint32_t* rank, hipStream_t s);
 // sets the mask bits of rlist[lo, hi); with tile_min, also the least q per tile
@@ -366,16 +369,10 @@ hipError_t configure_kernels();
 size_t resolve_lds_bytes(bool guided);
 size_t resolve_small_lds_bytes(bool guided);
 uint32_t small_batch_max();
-// items per workgroup of k_resolve / k_resolve_multi with `team` waves per item
+// items per workgroup of k_resolve (and per engine resolve unit and item turn) with `team` waves per item
 uint32_t items_per_wg(uint32_t team);
 // waves per item for a batch of n items (1, 2, 4 or 8)
 uint32_t team_for(uint64_t n, uint32_t team_max, uint32_t waves);
-// one round of the multi-job engine: units[u] is a job's batch, its workgroups
-// start at wg_start[u] (n_units + 1 entries), its items at item_start[u]
-hipError_t launch_resolve_multi(const BatchArgs* units, const uint32_t* wg_start, uint32_t n_units, uint32_t n_wg,
-                                bool guided, hipStream_t s);
-hipError_t launch_commit_multi(const BatchArgs* units, const uint32_t* item_start, uint32_t n_units, uint32_t total,
-                               hipStream_t s);
 // consecutive batches of at most small_batch_max() items, one workgroup
 // Consecutive batches (each at most wgs * 16 items) in one cooperative launch
 // of wgs workgroups of 16 waves; wgs <= coop_max_wgs(guided).

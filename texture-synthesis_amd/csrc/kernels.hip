@@ -49,12 +49,6 @@ __device__ __forceinline__ void wave_sync() {
 
 __device__ __forceinline__ uint64_t lanemask_lt(int lane) { return (1ull << lane) - 1ull; }
 
-__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
-    uint32_t lo = __shfl((int)(uint32_t)v, src);
-    uint32_t hi = __shfl((int)(uint32_t)(v >> 32), src);
-    return ((uint64_t)hi << 32) | lo;
-}
-
 // Canonical neighbour order: squared distance, then dy, then dx, ascending.
 __device__ __forceinline__ uint64_t make_key(int dx, int dy) {
     uint64_t d2 = (uint64_t)((int64_t)dx * dx + (int64_t)dy * dy);
@@ -1580,50 +1574,6 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve_flow(BatchArgs a
     launch_span(a, t_start);
 }
 
-// Index u with start[u] <= g < start[u + 1] (start[0] = 0, n entries).
-__device__ __forceinline__ uint32_t find_unit(const uint32_t* start, uint32_t n, uint32_t g) {
-    uint32_t lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (start[mid] <= g) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
-// One round of the multi-job engine: unit u is one job's current batch;
-// its workgroups are [wg_start[u], wg_start[u + 1]).
-template <bool GUIDED>
-__global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve_multi(const BatchArgs* __restrict__ units,
-                                                                      const uint32_t* __restrict__ wg_start,
-                                                                      uint32_t n_units) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ MapDesc smaps[kMapCache];
-    // the unit is uniform across the workgroup: keep its arguments scalar
-    const uint32_t u = (uint32_t)__builtin_amdgcn_readfirstlane((int)find_unit(wg_start, n_units, blockIdx.x));
-    const BatchArgs a = units[u];
-    float *lut, *glut;
-    const MapDesc* maps;
-    WaveLds* wls = setup_lds<GUIDED>(a, smem, lut, glut, smaps, maps);
-    const int wave = threadIdx.x / kWave;
-    const int lane = threadIdx.x % kWave;
-    const int T = a.team > 1 ? (int)a.team : 1;
-    const int team = wave / T, member = wave % T;
-    const uint32_t item = a.item0 + (blockIdx.x - wg_start[u]) * (uint32_t)(kWavesPerWG / T) + (uint32_t)team;
-    resolve_team<GUIDED>(a, maps, item, a.item0, a.n_res, wls + team * T, member, T, item < a.item1, lut, glut, lane);
-}
-
-// Commit of every unit of a round: thread g commits item g - item_start[u].
-__global__ void k_commit_multi(const BatchArgs* units, const uint32_t* item_start, uint32_t n_units,
-                               uint32_t total) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= total) return;
-    const uint32_t u = find_unit(item_start, n_units, g);
-    const BatchArgs& a = units[u];
-    commit_one(a.W, a.tiles_x, a.maps, a.work, a.result, a.item0 + (g - item_start[u]), a.item0, a.first_new,
-               a.state_w, a.mask_w, a.score_w, a.gmask);
-}
-
 // Neighbour lists of items [item0, item1) ahead of their batches, one
 // wavefront per item (the resolve kernels read them instead of searching).
 __device__ __forceinline__ void prefetch_item(const PrefetchArgs& a, WaveLds* L, uint32_t item, int lane) {
@@ -1803,16 +1753,6 @@ __device__ __forceinline__ void rand_item(const RandArgs& a, uint32_t item) {
 __global__ void k_rand(RandArgs a) {
     const uint32_t item = a.item0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (item < a.item1) rand_item(a, item);
-}
-
-// Random candidates of several jobs in one launch: thread g serves item
-// g - item_start[u] of unit u.
-__global__ void k_rand_multi(const RandArgs* units, const uint32_t* item_start, uint32_t n_units, uint32_t total) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= total) return;
-    const uint32_t u = find_unit(item_start, n_units, g);
-    const RandArgs& a = units[u];
-    rand_item(a, a.item0 + (g - item_start[u]));
 }
 
 __global__ void k_commit(CommitArgs a) {
@@ -2030,6 +1970,13 @@ __device__ __forceinline__ T ld_uniform(const T* p) {
 // device-scope release is not needed (it made the engine four times slower).
 __device__ __forceinline__ void unit_release() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
+// A spread launch's jobs have workgroups on every XCD: after the workgroup's
+// waves have released to the L2 and met at the barrier, one thread writes
+// the L2 back (device-scope release) before the unit is counted finished.
+__device__ __forceinline__ void unit_publish(bool spread) {
+    if (spread) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+}
+
 // Weight-exp sweep (test hook): out[i - i0] = (float)exp_dd(-1.0 * x_i), the
 // expression of the neighbour weights in item_geometry, for the arguments
 // x_i = t * 8 when i % 4 != 0, else 8 + t * 56, with t = ((i * 0x9E3779B97F4A7C15)
@@ -2088,8 +2035,11 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) __attribute__((amdgpu_waves_per
     __shared__ unsigned long long s_idle_since;
     int32_t* oc = offtab_lds();
     for (uint32_t t = threadIdx.x; t < kOffCache && t < e.offtab_n; t += blockDim.x) oc[t] = e.offtab[t];
+    // the control words this workgroup uses: its XCD's, or XCD 0's for a spread launch
+    const bool spread = (e.flags & kEngineSpread) != 0u;
+    const uint32_t home = spread ? 0u : xcc_id();
     if (threadIdx.x == 0) {
-        const uint32_t x = xcc_id();
+        const uint32_t x = home;
         const uint32_t j0 = e.ctl->job0[x], nj = e.ctl->job0[x + 1] - j0;
         s_key = 0;
         s_hint = nj ? j0 + (blockIdx.x / 8u) % nj : j0;
@@ -2105,7 +2055,7 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) __attribute__((amdgpu_waves_per
         __syncthreads();
         const uint64_t c0 = wall_clock64();  // scalar: no VGPRs held across the unit
         if (wave == 0) {
-            const uint32_t xcc = xcc_id();
+            const uint32_t xcc = home;
             const uint32_t j0 = e.ctl->job0[xcc], nj = e.ctl->job0[xcc + 1] - j0;
             const uint32_t hint = s_hint;
             uint64_t idle_since = s_idle_since;
@@ -2303,12 +2253,13 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) __attribute__((amdgpu_waves_per
         unit_release();
         __syncthreads();
         if (tid == 0) {
+            unit_publish(spread);
             const uint64_t c2 = wall_clock64();
             s_t[P.kind == Q_RESOLVE ? 1 : 2] += c2 - c1;
             s_nu[P.kind == Q_RESOLVE ? 0 : 1] += 1;
             const uint32_t old = __hip_atomic_fetch_add(&e.dyn[jb].finished, 1u, __ATOMIC_RELAXED,
                                                         __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&e.ctl->x[xcc_id()].progress, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&e.ctl->x[home].progress, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             s_last = old + 1u == P.units;
             if (s_last) unit_acquire();
         }
@@ -2329,9 +2280,10 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) __attribute__((amdgpu_waves_per
             if (tid == 0) {
                 __hip_atomic_store(&e.dyn[jb].finished, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 unit_release();
+                unit_publish(spread);
                 __hip_atomic_store(&e.dyn[jb].cursor, (unsigned long long)(phi + 1u) << 32, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
-                if (phi + 1u == J.n_phases) atomicSub(&e.ctl->x[xcc_id()].active, 1u);
+                if (phi + 1u == J.n_phases) atomicSub(&e.ctl->x[home].active, 1u);
                 s_t[3] += wall_clock64() - c3;
             }
         }
@@ -2379,12 +2331,6 @@ hipError_t configure_kernels() {
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)k_resolve_small<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)resolve_small_lds_bytes(true));
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void*)k_resolve_multi<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)resolve_lds_bytes(true));
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void*)k_resolve_multi<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)resolve_lds_bytes(false));
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)k_resolve_coop<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)resolve_small_lds_bytes(true));
@@ -2540,27 +2486,6 @@ uint32_t team_for(uint64_t n, uint32_t team_max, uint32_t waves) {
     return T > 0 ? T : 1;
 }
 
-hipError_t launch_resolve_multi(const BatchArgs* units, const uint32_t* wg_start, uint32_t n_units, uint32_t n_wg,
-                                bool guided, hipStream_t s) {
-    if (n_units == 0 || n_wg == 0) return hipSuccess;
-    size_t lds = resolve_lds_bytes(guided);
-    if (guided) {
-        hipLaunchKernelGGL(k_resolve_multi<true>, dim3(n_wg), dim3(kWave * kWavesPerWG), lds, s, units, wg_start,
-                           n_units);
-    } else {
-        hipLaunchKernelGGL(k_resolve_multi<false>, dim3(n_wg), dim3(kWave * kWavesPerWG), lds, s, units, wg_start,
-                           n_units);
-    }
-    return hipGetLastError();
-}
-
-hipError_t launch_commit_multi(const BatchArgs* units, const uint32_t* item_start, uint32_t n_units, uint32_t total,
-                               hipStream_t s) {
-    if (n_units == 0 || total == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_commit_multi, dim3((total + 255) / 256), dim3(256), 0, s, units, item_start, n_units, total);
-    return hipGetLastError();
-}
-
 uint32_t coop_max_wgs(bool guided) {
     int per_cu = 0, cus = 0, dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
@@ -2676,13 +2601,6 @@ hipError_t launch_resolve(const BatchArgs& a, bool guided, hipStream_t s) {
     } else {
         hipExtLaunchKernelGGL(k_resolve<false>, grid, block, lds, s, g_ev_start, g_ev_stop, 0u, a);
     }
-    return hipGetLastError();
-}
-
-hipError_t launch_rand_multi(const RandArgs* units, const uint32_t* item_start, uint32_t n_units, uint32_t total,
-                             hipStream_t s) {
-    if (n_units == 0 || total == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_rand_multi, dim3((total + 255) / 256), dim3(256), 0, s, units, item_start, n_units, total);
     return hipGetLastError();
 }
 

@@ -573,6 +573,9 @@ struct tsg_ctx {
     std::map<std::vector<uint64_t>, tsg_tuning> tunings;
 };
 
+// Threads drawing one job's pick-chain indices in a single-job run.
+constexpr unsigned kPickThreads = 12;
+
 struct tsg_job {
     tsg_ctx* ctx = nullptr;
     tsg_params prm{};
@@ -632,6 +635,7 @@ struct tsg_job {
     bool ahead_deep = false;
     Ahead& aset(int t) { return ahead[ahead_deep ? t : t % 2]; }
     bool in_engine = false;   // run by a tsg_batch (no single-job run structures)
+    tsg_batch* owner = nullptr;  // the batch holding the job, from submission until it is taken back
     uint64_t ahead_open = 0;  // resolved count at the open of the next stage to prepare
     int ahead_next = 0;       // next stage to prepare
     bool ahead_wanted = false;  // ahead_next is due (prepared once its picks are ready)
@@ -645,11 +649,14 @@ struct tsg_job {
     // under the work-queue engine: what was planned since the last launch
     struct Eng {
         std::vector<QPhase> prog;        // phases (arg: index into args)
-        std::vector<BatchArgs> args;     // resolve phases' arguments and the open stages' entries
+        std::vector<BatchArgs> args;     // the next launch's stage argument blocks
+        BatchArgs stage_a{};             // the open stage's block, as planned at its opening
         std::vector<uint32_t> n_at;      // prefetch windows' resolved counts (pre_base: offset)
         uint32_t stage_arg = 0;          // the open stage's entry in args
         uint32_t sharing = 1;            // jobs sharing the device when planned
         uint64_t weight = 0;             // work of the planned phases (XCD placement)
+        uint64_t items = 0;              // resolve items planned for the next launch
+        uint64_t quantum = 0;            // at most this many per launch (0: no bound)
         bool planned_all = false;        // every phase of the job is planned
         DBuf<float> luts;                // per stage: 512-entry cost tables
     } eng;
@@ -691,7 +698,7 @@ struct tsg_job {
     std::condition_variable pk_cv;
     int pk_state = 0;  // 0 not submitted, 1 queued, 2 computing, 3 finished
     bool pk_cancel = false;
-    unsigned pk_threads = 12;  // threads of one pick chain's index draws (capped by the host's cores)
+    unsigned pk_threads = kPickThreads;  // threads of one pick chain's index draws (capped by the host's cores)
     uint64_t pk_start_res = 0;
     uint64_t overall_total = 0, overall_cur = 0;
     hipEvent_t ev_run0 = nullptr, ev_run1 = nullptr;
@@ -1859,6 +1866,7 @@ struct tsg_batch {
         tsg_job* job;
         int status;
         bool failed;  // status set by the engine (planning error, cancellation)
+        bool alone;   // runs (ran) alone on the single-job path; status final once run
     };
     std::vector<Entry> active, pending, done, taken;
     uint64_t next_ticket = 1;
@@ -1872,7 +1880,10 @@ struct tsg_batch {
     uint64_t idle_ticks = 200000000ull;  // 2 s of the 100 MHz clock without progress on an XCD
     uint32_t wgs_cap = 0;                 // test hook: at most this many engine workgroups (0: all resident)
     double prof[16] = {};                 // accumulated QXcd clocks and counts, host phase times (tsg_batch_engine_profile)
-    uint32_t engine_flags = 0;            // QEngineArgs::flags
+    uint32_t engine_flags = 0;            // test hook: kEngineSpread forces spread launches
+    uint64_t quantum = 0;                 // resolve items per job per launch (0: no bound)
+    uint32_t alone_below = 0;             // jobs starting while fewer are in the batch run alone (0: never)
+    uint64_t alone_ahead = 0;             // device bytes an alone run may add for its stage-ahead sets
     tsg_tuning tune{};
     ~tsg_batch() {
         if (h_blk) (void)hipHostFree(h_blk);
@@ -1886,6 +1897,9 @@ constexpr uint32_t kEngineFuseMax = 2048;
 // Launches with at most this many jobs compute the neighbour lists of
 // latency-bound batches ahead, as a single-job run does.
 constexpr uint32_t kEnginePrefetchJobs = 8;
+// Launches of fewer jobs than this are spread over every XCD (home placement
+// would give a lone job one XCD in eight).
+constexpr size_t kSpreadBelow = 8;
 // Items per team in one resolve unit when more jobs than that share a launch
 // (256-job mix: 1, 2, 4, 8, 16 measured 1.22, 1.10, 1.02, 1.02, 1.10 s; the
 // waves of a unit wait for its slowest item, and larger units average that
@@ -1979,6 +1993,7 @@ int eng_open_stage(tsg_job* j) {
     }
     j->eng.stage_arg = (uint32_t)j->eng.args.size();
     j->eng.args.push_back(a);
+    j->eng.stage_a = a;
     // next_pyramid_level (ms.rs:687-700): every resolved pixel from the new level
     if (st.level > 0) eng_push(j, Q_RECOLOR, 0, (uint32_t)j->n_res, kQElems, kNoArg);
     st.open = true;
@@ -2022,6 +2037,12 @@ void eng_prefetch(tsg_job* j, uint64_t i0) {
 int eng_plan(tsg_job* j) {
     tsg_job::Step& st = j->step;
     int rc;
+    // a stage left open by the last launch (its quantum): this launch's
+    // phases of the stage need the stage's block among this launch's blocks
+    if (st.open && j->eng.args.empty()) {
+        j->eng.stage_arg = 0;
+        j->eng.args.push_back(j->eng.stage_a);
+    }
     while (!step_done(j)) {
         if (!st.open) {
             if (!picks_ready_now(j, st.t)) return TSG_OK;
@@ -2037,6 +2058,8 @@ int eng_plan(tsg_job* j) {
             st.t += 1;
             continue;
         }
+        // the launch's share of this job is planned (at least one batch)
+        if (j->eng.quantum && j->eng.items >= j->eng.quantum) return TSG_OK;
         const uint32_t m = (uint32_t)j->prm.random_sample_locations;
         if (!(i0 >= st.rc_lo && i1 <= st.rc_hi)) {
             const uint64_t cap_items = std::max<uint64_t>(i1 - i0, (uint64_t)j->rc_budget / (8ull * std::max(1u, m)));
@@ -2071,6 +2094,7 @@ int eng_plan(tsg_job* j) {
         q.fuse = n <= kEngineFuseMax;
         if (!q.fuse) eng_push(j, Q_COMMIT, (uint32_t)i0, (uint32_t)i1, kQElems, kNoArg);
         j->eng.weight += (uint64_t)n * (j->prm.nearest_neighbors + 8);
+        j->eng.items += n;
         step_advance(j, i0, i1);
     }
     j->eng.planned_all = true;
@@ -2097,6 +2121,7 @@ int eng_begin(tsg_job* j) {
     j->eng.n_at.clear();
     j->eng.stage_arg = 0;
     j->eng.weight = 0;
+    j->eng.items = 0;
     j->eng.planned_all = false;
     return TSG_OK;
 }
@@ -2121,9 +2146,59 @@ int batch_admit(tsg_batch* b) {
     while (b->active.size() < b->max_active && !b->pending.empty()) {
         tsg_batch::Entry e = b->pending.front();
         b->pending.erase(b->pending.begin());
+        // few jobs in the batch: this one gets the whole device to itself
+        e.alone = b->alone_below && b->active.size() + b->pending.size() + 1 < b->alone_below;
         b->active.push_back(e);
+        if (e.alone) continue;
         const int rc = eng_begin(e.job);
         if (rc) eng_drop(b, b->active.size() - 1, rc);
+    }
+    return TSG_OK;
+}
+
+// Jobs admitted to run alone: each runs to its end on the single-job path
+// (tsg_job_run's launch sequence), one after another, smallest first, before
+// the round's engine launch. Its stage-ahead sets (which the dependency-
+// ordered runs need) are used only when they fit the bytes the caller allows
+// beyond tsg_footprint_estimate, and are released when the run ends.
+int batch_run_alone(tsg_batch* b) {
+    std::vector<tsg_batch::Entry> runs;
+    for (size_t q = 0; q < b->active.size();) {
+        if (!b->active[q].alone) {
+            ++q;
+            continue;
+        }
+        runs.push_back(b->active[q]);
+        b->active.erase(b->active.begin() + q);
+    }
+    std::stable_sort(runs.begin(), runs.end(),
+                     [](const tsg_batch::Entry& x, const tsg_batch::Entry& y) { return x.job->N < y.job->N; });
+    for (tsg_batch::Entry& e : runs) {
+        tsg_job* j = e.job;
+        // a single-job run (its small-batch tables, all pick threads); the
+        // random candidates keep the engine's chunk size of the estimate
+        j->in_engine = false;
+        j->pk_threads = kPickThreads;
+        if (j->tune.ahead >= 2 && ahead_bytes_deep(j) > b->alone_ahead) j->tune.ahead = 1;
+        if (ahead_bytes(j) > b->alone_ahead) j->tune.ahead = 0;
+        e.status = run_job(j, nullptr, nullptr);
+        // the run has ended (run_job waited for it, and the side stream is
+        // drained here): its stage-ahead sets go back to the device cache
+        // now, not with the job
+        HIP_TRY(hipStreamSynchronize(j->ctx->side));
+        g_pool_release = true;
+        for (tsg_job::Ahead& A : j->ahead) {
+            A.rc.release();
+            A.nbrs.release();
+            A.cnt.release();
+            A.n_at.release();
+            A.mend.release();
+            A.mnew.release();
+            A.tmin.release();
+            A.dep.release();
+        }
+        g_pool_release = false;
+        b->done.push_back(e);
     }
     return TSG_OK;
 }
@@ -2147,14 +2222,28 @@ int batch_round(tsg_batch* b) {
         t_mark = t;
     };
     batch_admit(b);
+    batch_run_alone(b);
     lap(6);
     if (b->active.empty()) return TSG_OK;
     // plan every job as far as its picks allow; a job that cannot be planned leaves
     for (size_t q = 0; q < b->active.size();) {
         b->active[q].job->eng.sharing = (uint32_t)b->active.size();
+        b->active[q].job->eng.quantum = b->quantum;
         const int rc = eng_plan(b->active[q].job);
         if (rc) {
             eng_drop(b, q, rc);
+            continue;
+        }
+        ++q;
+    }
+    // every phase's arguments must be among its job's blocks of this launch
+    // (a planning error leaves with its job, before anything is launched)
+    for (size_t q = 0; q < b->active.size();) {
+        const tsg_job::Eng& E = b->active[q].job->eng;
+        bool ok = true;
+        for (const QPhase& p0 : E.prog) ok = ok && p0.arg < E.args.size();
+        if (!ok) {
+            eng_drop(b, q, fail(TSG_ERR_DEVICE, "engine phase without its arguments"));
             continue;
         }
         ++q;
@@ -2176,6 +2265,7 @@ int batch_round(tsg_batch* b) {
         std::vector<size_t> order[2];
         for (size_t q : runs) order[b->active[q].job->guided ? 1 : 0].push_back(q);
         std::vector<uint32_t> xcc_of(b->active.size(), 0);
+        bool spread[2] = {false, false};
         for (int kd = 0; kd < 2; ++kd) {
             std::stable_sort(order[kd].begin(), order[kd].end(), [&](size_t x, size_t y) {
                 return b->active[x].job->eng.weight > b->active[y].job->eng.weight;
@@ -2184,9 +2274,11 @@ int batch_round(tsg_batch* b) {
             if (!b->wgs[kd]) b->wgs[kd] = engine_max_wgs(kd == 1);
             const uint32_t wgs_kd = b->wgs_cap ? std::min(b->wgs_cap, b->wgs[kd]) : b->wgs[kd];
             const int n_x = (int)std::min<uint32_t>(8, std::max<uint32_t>(1, wgs_kd));
+            // fewer jobs than XCDs: every XCD serves every job (a spread launch)
+            spread[kd] = (b->engine_flags & kEngineSpread) || order[kd].size() < (size_t)kSpreadBelow;
             uint64_t load[8] = {};
             for (size_t q : order[kd]) {
-                const int x = (int)(std::min_element(load, load + n_x) - load);
+                const int x = spread[kd] ? 0 : (int)(std::min_element(load, load + n_x) - load);
                 load[x] += b->active[q].job->eng.weight + 1;
                 xcc_of[q] = (uint32_t)x;
             }
@@ -2284,7 +2376,7 @@ int batch_round(tsg_batch* b) {
             ea.offtab = b->ctx->offtab.p;
             ea.offtab_n = b->ctx->offtab_n;
             ea.idle_ticks = b->idle_ticks;
-            ea.flags = b->engine_flags;
+            ea.flags = spread[kd] ? kEngineSpread : 0u;
             const uint32_t wgs = b->wgs_cap ? std::min(b->wgs_cap, b->wgs[kd]) : b->wgs[kd];
             HIP_TRY(launch_engine(ea, reinterpret_cast<const BatchArgs*>(D + o_args), wgs, kd == 1, s));
             b->launches += 1;
@@ -2317,6 +2409,7 @@ int batch_round(tsg_batch* b) {
             E.args.clear();
             E.n_at.clear();
             E.weight = 0;
+            E.items = 0;
         }
         if (err[0] | err[1]) {
             // a workgroup saw no progress on its XCD for the idle limit, or a
@@ -2354,7 +2447,11 @@ int batch_reap(tsg_batch* b, bool wait) {
             ++q;
             continue;
         }
-        if (b->done[q].failed) {
+        if (b->done[q].alone) {
+            // run_job has waited for its end and set its state
+            j->step.active = false;
+            j->ran = true;
+        } else if (b->done[q].failed) {
             if (j->ev_run1) (void)hipEventSynchronize(j->ev_run1);
             j->step.active = false;
             j->ran = true;
@@ -2902,6 +2999,11 @@ int tsg_job_kernel_stats(tsg_job* job, double* ms, uint64_t* items, uint64_t* la
 
 void tsg_job_destroy(tsg_job* job) {
     if (!job) return;
+    if (job->owner) {
+        // still held by a batch, which destroys it (or hands it back with tsg_batch_take)
+        fail(TSG_ERR_INVALID_ARGUMENT, "the job belongs to a batch until it is taken back");
+        return;
+    }
     (void)hipSetDevice(job->ctx->device);
     // a finished job has no work in flight (downloads wait for theirs)
     if (job->step.active) {
@@ -3366,6 +3468,7 @@ int tsg_batch_submit(tsg_batch* b, tsg_job* job, uint64_t* ticket) {
         for (const tsg_batch::Entry& e : *v)
             if (e.job == job) return fail(TSG_ERR_INVALID_ARGUMENT, "job already submitted");
     *ticket = b->next_ticket++;
+    job->owner = b;
     b->pending.push_back({*ticket, job, 0, false});
     return TSG_OK;
 }
@@ -3398,6 +3501,19 @@ int tsg_debug_exp_sweep_host(uint64_t i0, uint64_t n, float* out) {
 int tsg_batch_engine_profile(tsg_batch* b, double* out11) {
     if (!b || !out11) return fail(TSG_ERR_INVALID_ARGUMENT, "null argument");
     for (int q = 0; q < 11; ++q) out11[q] = b->prof[q];
+    return TSG_OK;
+}
+
+int tsg_batch_set_quantum(tsg_batch* b, uint64_t items) {
+    if (!b) return fail(TSG_ERR_INVALID_ARGUMENT, "null batch");
+    b->quantum = items;
+    return TSG_OK;
+}
+
+int tsg_batch_set_alone_below(tsg_batch* b, uint32_t jobs, uint64_t ahead_bytes) {
+    if (!b) return fail(TSG_ERR_INVALID_ARGUMENT, "null batch");
+    b->alone_below = jobs;
+    b->alone_ahead = ahead_bytes;
     return TSG_OK;
 }
 
@@ -3481,6 +3597,7 @@ int tsg_batch_take(tsg_batch* b, uint64_t ticket, tsg_job** job) {
     for (size_t q = 0; q < b->taken.size(); ++q)
         if (b->taken[q].ticket == ticket) {
             *job = b->taken[q].job;
+            (*job)->owner = nullptr;
             b->taken.erase(b->taken.begin() + q);
             return TSG_OK;
         }
@@ -3500,7 +3617,10 @@ void tsg_batch_destroy(tsg_batch* b) {
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
     for (auto* v : {&b->active, &b->pending, &b->done, &b->taken})
-        for (auto& e : *v) tsg_job_destroy(e.job);
+        for (auto& e : *v) {
+            e.job->owner = nullptr;
+            tsg_job_destroy(e.job);
+        }
     delete b;
 }
 

@@ -167,6 +167,8 @@ def lib():
     L.tsg_batch_cancel.argtypes = [vp, u64]
     L.tsg_debug_fail_allocs.argtypes = [C.c_int64]
     L.tsg_batch_debug_engine.argtypes = [vp, u32, u32]
+    L.tsg_batch_set_quantum.argtypes = [vp, u64]
+    L.tsg_batch_set_alone_below.argtypes = [vp, u32, u64]
     L.tsg_batch_engine_profile.argtypes = [vp, vp]
     L.tsg_debug_exp_sweep.argtypes = [vp, u64, u64, vp]
     L.tsg_debug_exp_sweep_host.argtypes = [u64, u64, vp]
@@ -554,13 +556,29 @@ class Batch:
     in the reference, whose Session::run (session.rs:37-66) synthesises one
     image at a time."""
 
-    def __init__(self, ctx: Optional[Context] = None, max_active: int = 256):
+    def __init__(self, ctx: Optional[Context] = None, max_active: int = 256, quantum: int = 0,
+                 alone_below: int = 0):
+        """quantum: resolve items each session may plan per round (0: no
+        bound); a bound keeps rounds short for sessions submitted meanwhile.
+        alone_below: a session that starts while fewer than this many are in
+        the batch runs alone, with the whole device (0: never)."""
         self._ctx = ctx or Context.default()
+        self._sessions = {}
+        self._status = {}
         h = C.c_void_p()
         _check(lib().tsg_batch_create(self._ctx._h, int(max_active), C.byref(h)))
         self._h = h
-        self._sessions = {}
-        self._status = {}
+        if quantum:
+            _check(lib().tsg_batch_set_quantum(h, int(quantum)))
+        self._alone_below = int(alone_below)
+        if alone_below:
+            _check(lib().tsg_batch_set_alone_below(h, int(alone_below), 0))
+
+    def set_alone(self, below: int, ahead_bytes: int = 0):
+        """Sessions starting while fewer than `below` are in the batch run
+        alone; their stage-ahead buffers are used when they fit ahead_bytes."""
+        self._alone_below = int(below)
+        _check(lib().tsg_batch_set_alone_below(self._h, int(below), max(0, int(ahead_bytes))))
 
     def submit(self, session: Session) -> int:
         """Hands the session over; returns its ticket."""
@@ -568,8 +586,9 @@ class Batch:
             raise RuntimeError("a session runs once")
         t = C.c_uint64()
         _check(lib().tsg_batch_submit(self._h, session._job, C.byref(t)))
-        self._sessions[t.value] = session
+        # the batch owns the job from here on
         session._job = None
+        self._sessions[t.value] = session
         return t.value
 
     def step(self, max_rounds: int = 1) -> int:
@@ -591,7 +610,7 @@ class Batch:
                     launch_to_done_ms=v[9], host_finish_ms=v[10])
 
     def debug_engine(self, wgs_cap: int = 0, flags: int = 0):
-        """Test hook: cap the engine's workgroups; flags are reserved (0)."""
+        """Test hook: cap the engine's workgroups; flags bit 0 spreads every launch over all XCDs."""
         _check(lib().tsg_batch_debug_engine(self._h, int(wgs_cap), int(flags)))
 
     def cancel(self, ticket: int):

@@ -7,7 +7,9 @@ estimate fit the memory budget and an engine slot is free: it is then built
 ``Batch``, whose rounds run every admitted request's planned work in one
 launch of the work-queue engine. Requests that arrive while others run join
 at the next round, and a finished request frees its memory and slot for the
-next in line. A request whose estimate alone exceeds the budget is rejected
+next in line. Under light load (fewer than ALONE_BELOW requests in flight) a
+request starting runs alone with the whole device instead, which shortens
+its chain of small batches; the engine takes over as requests pile up. A request whose estimate alone exceeds the budget is rejected
 (AdmissionRejected) rather than held; a request can be cancelled; a request
 that fails (invalid, out of memory) is reported without disturbing others.
 
@@ -27,6 +29,10 @@ from . import Batch, Context, GeneratedImage, SessionBuilder, TsgError
 
 ADMISSION_REJECTED = -24
 CANCELLED = -23
+# resolve items per request per engine round (0: no bound)
+QUANTUM = 0
+# a request that starts while fewer than this many are in flight runs alone
+ALONE_BELOW = 8
 
 
 @dataclass
@@ -71,12 +77,20 @@ class Server:
     max_active: engine slots (requests advanced per round).
     rounds_per_pump: engine rounds run by each ``pump`` call before finished
     requests are collected and waiting ones admitted.
+    quantum: resolve items a request may plan per round; it bounds a round's
+    length, and so how long a new request waits for the next one.
+    alone_below: a request that starts while fewer than this many are in
+    flight runs alone, with the whole device; under load requests share the
+    work-queue engine.
     """
 
     def __init__(self, ctx: Optional[Context] = None, hbm_budget: int = 64 << 30, max_active: int = 256,
-                 rounds_per_pump: int = 8, clock: Callable[[], float] = time.perf_counter):
+                 rounds_per_pump: int = 1, quantum: int = QUANTUM, alone_below: int = ALONE_BELOW,
+                 clock: Callable[[], float] = time.perf_counter):
         self.ctx = ctx or Context.default()
-        self.batch = Batch(self.ctx, max_active=max_active)
+        self.batch = Batch(self.ctx, max_active=max_active, quantum=quantum, alone_below=alone_below)
+        self.alone_below = int(alone_below)
+        self.quantum = int(quantum)
         self.hbm_budget = int(hbm_budget)
         self.max_active = int(max_active)
         self.rounds_per_pump = int(rounds_per_pump)
@@ -153,6 +167,9 @@ class Server:
         """Admits what fits, runs engine rounds, returns the requests that finished."""
         done = self._admit()
         if self.running:
+            # a request running alone may use the budget's free bytes for its
+            # stage-ahead buffers (released when its run ends)
+            self.batch.set_alone(self.alone_below, self.hbm_budget - self.in_use)
             self.batch.step(self.rounds_per_pump if rounds is None else rounds)
         for ticket, status in self.batch.poll():
             r = self.running.pop(ticket)
