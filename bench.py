@@ -24,8 +24,13 @@ Two more legs report in the same line:
   over 128^2..2048^2 from seeded 256^2 examples, 30% inpainting with 1-3
   elliptical holes covering 20-40% (the example resized to the output with
   CatmullRom); admission by device-memory footprint; p50/p99 latency
-  (arrival to result on the host) and output Mpix/s. A second, shorter run
-  under a tight budget shows admission held back by memory.
+  (arrival to result on the host) and output Mpix/s. Requests starting
+  while fewer than 8 are in flight run alone with the whole device; as they
+  pile up they share the work-queue engine (the line says how many took
+  which path). A third run at a higher rate ("serving_loaded") shows the
+  engine taking over, and a second at that rate under a tight budget shows
+  admission held back by memory. The requests' images are generated before the
+  clock starts.
 """
 import hashlib
 import argparse
@@ -162,6 +167,12 @@ def batched_leg(ts, synth, n, sequential=True):
     st = batch.stats()
     prof = batch.engine_profile()
     batch.close()
+    # algorithmic bytes of the engine launches: every job's resolves (its
+    # stage sizes, (0.5^s N) for s = 0..5) at 4k(k+m) + 24k + 28 bytes each
+    alg = sum(sum(int((0.5 ** s_) * o * o) for s_ in range(6)) * (4 * k * (k + M) + 24 * k + 28)
+              for _, o, k, _ in specs)
+    t_kern = prof["launch_to_done_ms"] / 1e3
+    ec = counters_from_profiles("engine_c4_counters.md", "k_engine")
     return {
         "workload": f"{n} syntheses: examples uniform in {{64,128,256,512}}^2, outputs uniform in "
                     f"{{128,256,512,1024}}^2, k in {{20,50}}, m=50, seeds 4000+j",
@@ -175,6 +186,13 @@ def batched_leg(ts, synth, n, sequential=True):
         "identical_to_alone": (seq == got) if seq is not None else None,
         "launches": st["launches"], "engine_rounds": st["rounds"],
         "engine_workgroup_ms": {k: (round(v, 1) if isinstance(v, float) else v) for k, v in prof.items()},
+        "roofline": {"kernel": "k_engine<false>", "bound": "hbm", "algorithmic_bytes": alg,
+                     "achieved": round(alg / t_kern / 1e9, 1) if t_kern > 0 else None, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(alg / t_kern / 1e9 / HBM_PEAK_GBS, 4) if t_kern > 0 else None,
+                     "over": "launch to completion of the engine launches (host clock)",
+                     "traffic": (ec or {}).get("hbm_bytes"), "l2_hit": (ec or {}).get("l2_hit"),
+                     "valu_busy": (ec or {}).get("valu_busy"), "waves_per_cu": (ec or {}).get("waves_per_cu"),
+                     "limiter": limiter(ec), "counters_from": "profiles/engine_c4_counters.md"},
     }
 
 
@@ -258,7 +276,7 @@ def serving_leg(ts, synth, n, qps, budget_gib, quantum=None, rounds_per_pump=1):
         fin.extend(srv.pump())
         busy += time.perf_counter() - ta
     t_end = time.perf_counter()
-    launches = srv.batch.stats()["launches"]
+    bst = srv.batch.stats()
     srv.close()
     errors = [f for f in fin if f.error is not None]
     lat = [f.latency_s * 1e3 for f in fin if f.error is None]
@@ -274,7 +292,9 @@ def serving_leg(ts, synth, n, qps, budget_gib, quantum=None, rounds_per_pump=1):
         "busy_fraction": round(busy / (t_end - t0), 3),
         "hbm_budget_gib": budget_gib, "peak_admitted_gib": round(srv.peak_in_use / (1 << 30), 3),
         "memory_holds": srv.memory_holds, "rejected": srv.rejected,
-        "engine": {"quantum_items": srv.quantum, "rounds_per_pump": rounds_per_pump, "launches": launches},
+        "engine": {"launches": bst["launches"], "requests_run_alone": bst["alone"],
+                   "requests_on_engine": len(fin) - bst["alone"] - len(errors), "alone_below": srv.alone_below,
+                   "quantum_items": srv.quantum, "rounds_per_pump": rounds_per_pump},
     }
 
 
@@ -289,6 +309,30 @@ def traffic_from_profiles():
     with open(path) as f:
         m = re.search(r"^hbm_bytes_per_launch:\s*([0-9.]+)", f.read(), re.M)
     return float(m.group(1)) if m else None
+
+
+def counters_from_profiles(name, kernel):
+    """Counter figures of `kernel` from a committed summary under profiles/
+    (lines "<figure>_<kernel>: value"), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    out = {}
+    with open(path) as f:
+        for m in re.finditer(r"^([a-z0-9_]+)_" + re.escape(kernel) + r":\s*([0-9.]+)", f.read(), re.M):
+            out[m.group(1)] = float(m.group(2))
+    return out or None
+
+
+def limiter(c):
+    """The limiter the counters point at."""
+    if not c:
+        return None
+    if c.get("valu_busy", 0) >= 0.5:
+        return "vector ALU (VALU busy %.0f%%), then memory latency" % (100 * c["valu_busy"])
+    return "memory latency: waves wait %.0f%% of their cycles, VALU busy %.0f%%, %.1f resident waves per CU" % (
+        100 * c.get("wait_fraction", 0), 100 * c.get("valu_busy", 0), c.get("waves_per_cu", 0))
 
 
 def copy_calibration(mb):
@@ -326,6 +370,10 @@ def main():
     ap.add_argument("--serve-tight-requests", type=int, default=300,
                     help="requests of a second serving run under a tight memory budget (0 = skip)")
     ap.add_argument("--serve-tight-gib", type=float, default=0.5)
+    ap.add_argument("--serve-loaded-requests", type=int, default=400,
+                    help="requests of a third serving run at --serve-loaded-qps, where requests pile up "
+                         "and share the work-queue engine (0 = skip)")
+    ap.add_argument("--serve-loaded-qps", type=float, default=30.0)
     ap.add_argument("--serve-qps", type=float, default=18.0,
                     help="Poisson arrival rate; 18/s keeps the engine busy about 70%% of the time")
     ap.add_argument("--serve-budget-gib", type=float, default=4.0)
@@ -387,6 +435,7 @@ def main():
     per_launch = res["items"] * BYTES_PER_RESOLVE / max(1, res["launches"])
     ms_per_step = t_all * 1e3 / args.steps
     value = 1024 * 1024 * args.steps / t_all / 1e6
+    rc_ctr = counters_from_profiles("resolve_c2_counters.md", "k_resolve")
     lat_sorted = sorted(lat)
     p99 = lat_sorted[min(len(lat_sorted) - 1, int(round(0.99 * (len(lat_sorted) - 1))))]
     out = {
@@ -414,6 +463,9 @@ def main():
                                    "resolved list downloaded to host memory"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
+                     "l2_hit": (rc_ctr or {}).get("l2_hit"), "lds_bank_conflict": (rc_ctr or {}).get("lds_bank_conflict"),
+                     "valu_busy": (rc_ctr or {}).get("valu_busy"), "waves_per_cu": (rc_ctr or {}).get("waves_per_cu"),
+                     "limiter": limiter(rc_ctr), "counters_from": "profiles/resolve_c2_counters.md",
                      "kernel": "k_resolve_persist + k_resolve", "bytes_per_resolve": BYTES_PER_RESOLVE,
                      "algorithmic_bytes_per_launch": round(per_launch),
                      "avg_launch_ms": round(res["ms"] / max(1, res["launches"]), 4),
@@ -429,8 +481,11 @@ def main():
     if args.serve_requests:
         out["serving"] = serving_leg(ts, synth, args.serve_requests, args.serve_qps, args.serve_budget_gib)
     if args.serve_tight_requests:
-        out["serving_tight_budget"] = serving_leg(ts, synth, args.serve_tight_requests, args.serve_qps,
+        out["serving_tight_budget"] = serving_leg(ts, synth, args.serve_tight_requests, args.serve_loaded_qps,
                                                   args.serve_tight_gib)
+    if args.serve_loaded_requests:
+        out["serving_loaded"] = serving_leg(ts, synth, args.serve_loaded_requests, args.serve_loaded_qps,
+                                            args.serve_budget_gib)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, host_cpu()[1])
         out["cpu_baseline"] = cpu_baseline(ex, threads)

@@ -301,7 +301,9 @@ int tsg_batch_run_all(tsg_batch* b);
 /* Finished jobs not yet taken: tickets and their run status (0 = ok). */
 int tsg_batch_poll(tsg_batch* b, uint64_t* tickets, int* status, uint32_t cap, uint32_t* n);
 int tsg_batch_take(tsg_batch* b, uint64_t ticket, tsg_job** job);
-int tsg_batch_stats(tsg_batch* b, uint64_t* rounds, uint64_t* launches, uint64_t* units);
+/* Rounds and engine launches so far, engine phases run, and jobs run alone
+ * (tsg_batch_set_alone_below); any pointer may be NULL. */
+int tsg_batch_stats(tsg_batch* b, uint64_t* rounds, uint64_t* launches, uint64_t* units, uint64_t* alone);
 /* Cancels a job that has not finished: a waiting job never starts, a
  * running one stops between launches. Either is then reported finished by
  * tsg_batch_poll with status TSG_ERR_CANCELLED and taken back as usual (its

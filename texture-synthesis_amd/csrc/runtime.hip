@@ -1884,6 +1884,7 @@ struct tsg_batch {
     uint64_t quantum = 0;                 // resolve items per job per launch (0: no bound)
     uint32_t alone_below = 0;             // jobs starting while fewer are in the batch run alone (0: never)
     uint64_t alone_ahead = 0;             // device bytes an alone run may add for its stage-ahead sets
+    uint64_t alone_runs = 0;              // jobs run alone so far
     tsg_tuning tune{};
     ~tsg_batch() {
         if (h_blk) (void)hipHostFree(h_blk);
@@ -2182,6 +2183,7 @@ int batch_run_alone(tsg_batch* b) {
         if (j->tune.ahead >= 2 && ahead_bytes_deep(j) > b->alone_ahead) j->tune.ahead = 1;
         if (ahead_bytes(j) > b->alone_ahead) j->tune.ahead = 0;
         e.status = run_job(j, nullptr, nullptr);
+        b->alone_runs += 1;
         // the run has ended (run_job waited for it, and the side stream is
         // drained here): its stage-ahead sets go back to the device cache
         // now, not with the job
@@ -3604,11 +3606,12 @@ int tsg_batch_take(tsg_batch* b, uint64_t ticket, tsg_job** job) {
     return fail(TSG_ERR_INVALID_ARGUMENT, "ticket not finished or unknown");
 }
 
-int tsg_batch_stats(tsg_batch* b, uint64_t* rounds, uint64_t* launches, uint64_t* units) {
+int tsg_batch_stats(tsg_batch* b, uint64_t* rounds, uint64_t* launches, uint64_t* units, uint64_t* alone) {
     if (!b) return fail(TSG_ERR_INVALID_ARGUMENT, "null batch");
     if (rounds) *rounds = b->rounds;
     if (launches) *launches = b->launches;
     if (units) *units = b->units_total;
+    if (alone) *alone = b->alone_runs;
     return TSG_OK;
 }
 

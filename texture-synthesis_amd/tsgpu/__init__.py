@@ -163,7 +163,7 @@ def lib():
     L.tsg_batch_run_all.argtypes = [vp]
     L.tsg_batch_poll.argtypes = [vp, vp, vp, u32, P(u32)]
     L.tsg_batch_take.argtypes = [vp, u64, P(vp)]
-    L.tsg_batch_stats.argtypes = [vp, P(u64), P(u64), P(u64)]
+    L.tsg_batch_stats.argtypes = [vp, P(u64), P(u64), P(u64), P(u64)]
     L.tsg_batch_cancel.argtypes = [vp, u64]
     L.tsg_debug_fail_allocs.argtypes = [C.c_int64]
     L.tsg_batch_debug_engine.argtypes = [vp, u32, u32]
@@ -647,9 +647,9 @@ class Batch:
             s._job = None
 
     def stats(self):
-        r, la, u = C.c_uint64(), C.c_uint64(), C.c_uint64()
-        _check(lib().tsg_batch_stats(self._h, C.byref(r), C.byref(la), C.byref(u)))
-        return dict(rounds=r.value, launches=la.value, units=u.value)
+        r, la, u, al = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(lib().tsg_batch_stats(self._h, C.byref(r), C.byref(la), C.byref(u), C.byref(al)))
+        return dict(rounds=r.value, launches=la.value, units=u.value, alone=al.value)
 
     def close(self):
         if getattr(self, "_h", None) and _lib is not None:
