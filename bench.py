@@ -80,10 +80,13 @@ def host_cpu():
 
 
 def cpu_baseline(ex, threads, runs=3):
-    """The oracle on the host's cores: the full 1024^2 workload under the
-    same Jacobi schedule on `threads` threads (median of `runs`), and the
-    reference's own CPU case, 128^2 -> 256^2 at one thread (one pixel per
-    batch: the reference's max_thread_count(1) semantics, seed 211)."""
+    """The oracle on the host's cores, with the reference's own threading
+    (ms.rs:815-1036: one worker while fewer than 1000 items are to be redone,
+    then `threads` workers with immediate updates and the TreeGrid kNN) on
+    the full 1024^2 workload, median of `runs`; beside it the same workload
+    under the device's Jacobi schedule on `threads` threads (one run), and
+    the reference's own CPU case, 128^2 -> 256^2 at one thread (one pixel
+    per batch: the reference's max_thread_count(1) semantics, seed 211)."""
     import tsgpu as ts
     from oracle import refcpu as R
     from tsgpu import synth
@@ -91,9 +94,13 @@ def cpu_baseline(ex, threads, runs=3):
     for _ in range(runs):
         job = R.Job(R.default_params(seed=0), [ex], 1024, 1024)
         t0 = time.perf_counter()
-        job.run(threads=threads, **ts.default_schedule())
+        job.run_reference_threads(threads)
         ts_all.append(time.perf_counter() - t0)
     dt = statistics.median(ts_all)
+    job = R.Job(R.default_params(seed=0), [ex], 1024, 1024)
+    t0 = time.perf_counter()
+    job.run(threads=threads, **ts.default_schedule())
+    t_jac = time.perf_counter() - t0
     c1 = synth.example(211, 128, 128)
     t1 = []
     for _ in range(runs):
@@ -105,8 +112,12 @@ def cpu_baseline(ex, threads, runs=3):
     model, avail = host_cpu()
     return dict(value=round(1024 * 1024 / dt / 1e6, 4), unit="Mpix/s", cores=threads, kind="port",
                 cpu_model=model, cores_available=avail,
-                sample=f"full 1024^2 syntheses of the same workload on {threads} host threads (oracle, same "
-                       f"Jacobi schedule), median of {runs}: " + ", ".join(f"{t:.2f}" for t in ts_all) + " s",
+                sample=f"full 1024^2 syntheses of the same workload, the oracle with the reference's threading "
+                       f"(serial while fewer than 1000 items are to be redone, then {threads} workers, immediate "
+                       f"updates, TreeGrid kNN), median of {runs}: " + ", ".join(f"{t:.2f}" for t in ts_all) + " s",
+                jacobi_schedule={"value": round(1024 * 1024 / t_jac / 1e6, 4), "unit": "Mpix/s", "cores": threads,
+                                 "seconds": round(t_jac, 2),
+                                 "note": "the same workload under the device's Jacobi batch schedule"},
                 one_thread_c1={"value": round(256 * 256 / d1 / 1e6, 4), "unit": "Mpix/s", "cores": 1,
                                "workload": "128^2 sinusoid+noise example (seed 211) -> 256^2, one pixel per batch "
                                            "(max_thread_count(1) semantics), seed 211",

@@ -27,6 +27,9 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <memory>
+#include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -577,8 +580,19 @@ void weights_of(const orc_job* j, Scratch& s, int32_t x, int32_t y) {
     for (size_t q = 0; q < kf; ++q) s.w[q] = (float)std::exp(-1.0 * s.dist[q * 4]);
 }
 
+/* A neighbour's record as find_candidates reads it: source coordinate and
+ * patch id / map (coord_map and id_map of ms.rs:207-217). */
+struct NbrRec {
+    uint32_t cx, cy, pid, pm;
+};
+struct PlainRead {
+    const orc_job* j;
+    NbrRec operator()(uint32_t f) const { return {j->cx[f], j->cy[f], j->pid[f], j->pm[f]}; }
+};
+
 /* find_candidates (ms.rs:478-602) */
-void find_candidates(const orc_job* j, Scratch& s, int32_t px, int32_t py, uint64_t m_seed) {
+template <class Rd>
+void find_candidates(const orc_job* j, Scratch& s, int32_t px, int32_t py, uint64_t m_seed, Rd rd) {
     size_t kf = s.nx.size();
     size_t count = 0;
     int32_t W = (int32_t)j->W, H = (int32_t)j->H;
@@ -587,8 +601,9 @@ void find_candidates(const orc_job* j, Scratch& s, int32_t px, int32_t py, uint6
     for (size_t q = 0; q < kf; ++q) {
         int32_t shx = px - s.nx[q], shy = py - s.ny[q];
         uint32_t nflat = (uint32_t)modulo(s.ny[q], H) * (uint32_t)W + (uint32_t)modulo(s.nx[q], W);
-        int32_t ccx = (int32_t)j->cx[nflat] + shx, ccy = (int32_t)j->cy[nflat] + shy;
-        uint32_t npid = j->pid[nflat], nmap = j->pm[nflat];
+        const NbrRec nr = rd(nflat);
+        int32_t ccx = (int32_t)nr.cx + shx, ccy = (int32_t)nr.cy + shy;
+        uint32_t npid = nr.pid, nmap = nr.pm;
         if (coord_valid(j, ccx, ccy, nmap, s.error)) {
             Cand& c = s.cands[count];
             c.kx.resize(kf);
@@ -719,7 +734,7 @@ void evaluate(const orc_job* j, Scratch& s, uint32_t flat, bool is_new, uint64_t
     }
     r.fallback = false;
     weights_of(j, s, x, y);
-    find_candidates(j, s, x, y, loop_seed + 1);
+    find_candidates(j, s, x, y, loop_seed + 1, PlainRead{j});
     /* own pattern from the output colour map; wrap iff tiling */
     {
         size_t kf = s.nx.size();
@@ -937,6 +952,324 @@ bool advance(orc_job* j, uint32_t stop_t, uint64_t stop_item) {
     j->finished = true;
     return true;
 }
+
+/* ------------------------------------------------------------------ */
+/* The reference's own threading (timing mode of the CPU baseline)     */
+/* ------------------------------------------------------------------ */
+/* ms.rs:815-1036: per stage, one worker while fewer than 1000 items are to
+ * be redone, else N workers taking items from an atomic counter. Each item
+ * reads the shared maps as they are and writes its result at once (the
+ * reference's unsynchronised UnsafeCell state; here relaxed atomic word
+ * accesses, so the restatement has no data race), and its k nearest come
+ * from a TreeGrid (ms.rs:1313-1530): a grid of cells, each searched exactly
+ * under a reader-writer lock (the reference's RwLock<RTree>), the query's
+ * cell and its 8 neighbours with the reference's boundary pruning, rebuilt
+ * with cells of sqrt(k W H / redo) * 2 + 1 pixels at the first fan-out.
+ * Results depend on thread timing, as the reference's do; with one thread
+ * the grid is one cell over the image (exact search, ties in the declared
+ * (d^2, dy, dx) order) and the run equals one pixel per batch. */
+
+namespace {
+
+inline uint32_t ld32(const uint32_t* p) { return __atomic_load_n(p, __ATOMIC_RELAXED); }
+inline void st32(uint32_t* p, uint32_t v) { __atomic_store_n(p, v, __ATOMIC_RELAXED); }
+
+struct LiveRead {
+    const orc_job* j;
+    NbrRec operator()(uint32_t f) const {
+        return {ld32(&j->cx[f]), ld32(&j->cy[f]), ld32(&j->pid[f]), ld32(&j->pm[f])};
+    }
+};
+
+/* One cell of the TreeGrid: its points bucketed by 8x8 pixels; exact k
+ * nearest by rings of buckets around the query. */
+struct GridCell {
+    std::shared_mutex mu;
+    int32_t x0 = 0, y0 = 0, nb = 1;  /* first pixel (grid coordinates) and buckets per side */
+    std::vector<std::vector<uint64_t>> b;
+    void init(int32_t gx0, int32_t gy0, int32_t size) {
+        x0 = gx0;
+        y0 = gy0;
+        nb = (size + 7) / 8;
+        b.assign((size_t)nb * nb, {});
+    }
+    void insert(int32_t gx, int32_t gy, int32_t x, int32_t y) {
+        int32_t bx = std::min(nb - 1, std::max(0, (gx - x0) >> 3)), by = std::min(nb - 1, std::max(0, (gy - y0) >> 3));
+        b[(size_t)by * nb + bx].push_back(((uint64_t)(uint32_t)x << 32) | (uint32_t)y);
+    }
+    /* the k nearest of (x, y) in this cell, ascending by (d^2, dy, dx), appended to out */
+    void knn(int32_t gx, int32_t gy, int32_t x, int32_t y, uint32_t k, std::vector<KnnEntry>& out) const {
+        std::vector<KnnEntry> best;
+        auto worse = [](const KnnEntry& a, const KnnEntry& c) { return key_less(a.d2, a.dy, a.dx, c.d2, c.dy, c.dx); };
+        const int32_t bx = std::min(nb - 1, std::max(0, (gx - x0) >> 3)), by = std::min(nb - 1, std::max(0, (gy - y0) >> 3));
+        for (int32_t r = 0; r < nb; ++r) {
+            if (best.size() >= k) {
+                /* every point of ring r is at least (r - 1) * 8 pixels away */
+                const int64_t lb = (int64_t)std::max(0, (r - 1) * 8);
+                if (lb * lb > best.front().d2) break;
+            }
+            for (int32_t yy = by - r; yy <= by + r; ++yy) {
+                if (yy < 0 || yy >= nb) continue;
+                const bool edge = yy == by - r || yy == by + r;
+                for (int32_t xx = bx - r; xx <= bx + r; xx += edge ? 1 : 2 * r) {
+                    if (xx >= 0 && xx < nb)
+                        for (uint64_t pt : b[(size_t)yy * nb + xx]) {
+                            const int32_t px = (int32_t)(uint32_t)(pt >> 32), py = (int32_t)(uint32_t)pt;
+                            const int64_t dx = px - x, dy = py - y;
+                            const KnnEntry e{dx * dx + dy * dy, (int32_t)dy, (int32_t)dx};
+                            if (best.size() < k) {
+                                best.push_back(e);
+                                std::push_heap(best.begin(), best.end(), worse);
+                            } else if (worse(e, best.front())) {
+                                std::pop_heap(best.begin(), best.end(), worse);
+                                best.back() = e;
+                                std::push_heap(best.begin(), best.end(), worse);
+                            }
+                        }
+                    if (r == 0) break;
+                }
+            }
+        }
+        std::sort_heap(best.begin(), best.end(), worse);
+        out.insert(out.end(), best.begin(), best.end());
+    }
+};
+
+/* TreeGrid (ms.rs:1313-1530) */
+struct TreeGridR {
+    int32_t gw = 1, gh = 1, ox = 0, oy = 0, chunk = 1;
+    std::vector<GridCell> cells;
+    TreeGridR(uint32_t width, uint32_t height, uint32_t chunk_size, uint32_t offset_x, uint32_t offset_y)
+        : cells(std::max<uint32_t>((width + chunk_size - 1) / chunk_size, 1) *
+                std::max<uint32_t>((height + chunk_size - 1) / chunk_size, 1)) {
+        gw = (int32_t)std::max<uint32_t>((width + chunk_size - 1) / chunk_size, 1);
+        gh = (int32_t)std::max<uint32_t>((height + chunk_size - 1) / chunk_size, 1);
+        ox = (int32_t)offset_x;
+        oy = (int32_t)offset_y;
+        chunk = (int32_t)chunk_size;
+        for (int32_t cx = 0; cx < gw; ++cx)
+            for (int32_t cy = 0; cy < gh; ++cy) cells[(size_t)cx * gh + cy].init(cx * chunk, cy * chunk, chunk);
+    }
+    GridCell* cell_of(int32_t x, int32_t y) {
+        const int64_t cx = (int64_t)(uint32_t)(x + ox) / chunk, cy = (int64_t)(uint32_t)(y + oy) / chunk;
+        if (cx >= gw || cy >= gh) return nullptr;  /* the reference would index out of its vector */
+        return &cells[(size_t)cx * gh + (size_t)cy];
+    }
+    void insert(int32_t x, int32_t y) {
+        GridCell* c = cell_of(x, y);
+        if (!c) return;
+        std::unique_lock<std::shared_mutex> lk(c->mu);
+        c->insert(x + ox, y + oy, x, y);
+    }
+    void knn(uint32_t x, uint32_t y, uint32_t k, std::vector<int32_t>& nx, std::vector<int32_t>& ny) {
+        nx.clear();
+        ny.clear();
+        const int32_t gx = (int32_t)x + ox, gy = (int32_t)y + oy;
+        const int32_t cx = gx / chunk, cy = gy / chunk;
+        const int64_t X = x, Y = y, C = chunk;
+        struct Place {
+            int32_t x, y;
+            bool center;
+            int64_t bx, by;
+        };
+        const Place places[9] = {
+            {cx, cy, true, 0, 0},
+            {cx + 1, cy, false, (cx + 1) * C - ox, Y},
+            {cx - 1, cy, false, cx * C - ox, Y},
+            {cx, cy - 1, false, X, cy * C - oy},
+            {cx, cy + 1, false, X, (cy + 1) * C - oy},
+            {cx + 1, cy + 1, false, (cx + 1) * C - ox, (cy + 1) * C - oy},
+            {cx - 1, cy + 1, false, cx * C - ox, (cy + 1) * C - oy},
+            {cx + 1, cy - 1, false, (cx + 1) * C - ox, cy * C - oy},
+            {cx - 1, cy - 1, false, cx * C - ox, cy * C - oy},
+        };
+        std::vector<KnnEntry> tmp;
+        tmp.reserve((size_t)k * 9);
+        int64_t ub = INT64_MAX;
+        for (const Place& pl : places) {
+            if (pl.x < 0 || pl.x >= gw || pl.y < 0 || pl.y >= gh) continue;
+            if (!pl.center) {
+                const int64_t d2 = (X - pl.bx) * (X - pl.bx) + (Y - pl.by) * (Y - pl.by);
+                if (d2 > ub) continue;
+            }
+            GridCell& c = cells[(size_t)pl.x * gh + pl.y];
+            {
+                std::shared_lock<std::shared_mutex> lk(c.mu);
+                c.knn(gx, gy, (int32_t)x, (int32_t)y, k, tmp);
+            }
+            /* the reference's approximation of the k-th best distance */
+            if (tmp.size() >= k && tmp.back().d2 < ub) ub = tmp.back().d2;
+        }
+        std::stable_sort(tmp.begin(), tmp.end(),
+                         [](const KnnEntry& a, const KnnEntry& b) { return key_less(a.d2, a.dy, a.dx, b.d2, b.dy, b.dx); });
+        for (size_t i = 0; i < tmp.size() && i < k; ++i) {
+            nx.push_back((int32_t)x + tmp[i].dx);
+            ny.push_back((int32_t)y + tmp[i].dy);
+        }
+    }
+};
+
+/* update (ms.rs:334-377): the maps at once; with update_list, flush_resolved
+ * (ms.rs:296-331) into the grid and the worker's own list */
+void live_update(orc_job* j, TreeGridR& g, std::vector<std::pair<uint32_t, float>>& mine, uint32_t flat, uint32_t sx,
+                 uint32_t sy, uint32_t smap, uint32_t patch, uint32_t pmap, bool update_list, float score,
+                 bool tiling) {
+    st32(&j->cx[flat], sx);
+    st32(&j->cy[flat], sy);
+    st32(&j->cm[flat], smap);
+    st32(&j->pid[flat], patch);
+    st32(&j->pm[flat], pmap);
+    uint32_t c;
+    memcpy(&c, j->maps[smap]->at(sx, sy), 4);
+    st32(reinterpret_cast<uint32_t*>(&j->color[(size_t)flat * 4]), c);
+    if (!update_list) return;
+    const int32_t x = (int32_t)(flat % j->W), y = (int32_t)(flat / j->W);
+    g.insert(x, y);
+    if (tiling) {
+        const int32_t W = (int32_t)j->W, H = (int32_t)j->H;
+        const int32_t xl = (int32_t)((float)W * 0.05f), xr = W - xl, yb = (int32_t)((float)H * 0.05f), yt = H - yb;
+        if (x < xl) g.insert(x + W, y);
+        else if (x > xr) g.insert(x - W, y);
+        if (y < yb) g.insert(x, y + H);
+        else if (y > yt) g.insert(x, y - H);
+    }
+    mine.push_back({flat, score});
+}
+
+bool run_reference_threads(orc_job* j, unsigned max_workers) {
+    if (!j->initialized && !init_run(j)) return false;
+    const bool tiling = j->prm.tiling_mode != 0;
+    const uint32_t W = j->W, H = j->H, k = j->prm.nearest_neighbors;
+    const uint32_t tw = (uint32_t)((float)W * std::fma(0.05f, 2.0f, 1.0f)) + 1;
+    const uint32_t th = (uint32_t)((float)H * std::fma(0.05f, 2.0f, 1.0f)) + 1;
+    const uint32_t offx = (uint32_t)((float)W * 0.05f) + 1, offy = (uint32_t)((float)H * 0.05f) + 1;
+    /* the resolve prelude (ms.rs:747-779): one cell over the image */
+    std::unique_ptr<TreeGridR> grid(new TreeGridR(tw, th, std::max(tw, th), offx, offy));
+    std::vector<uint32_t> all_points;  /* everything inserted, to rebuild the grid at the fan-out */
+    {
+        std::vector<std::pair<uint32_t, float>> sink;
+        std::vector<uint32_t> prev = j->resolved;
+        std::vector<float> prev_s = j->rscore;
+        j->resolved.clear();
+        j->rscore.clear();
+        for (size_t q = 0; q < prev.size(); ++q) {
+            const int32_t x = (int32_t)(prev[q] % W), y = (int32_t)(prev[q] / W);
+            grid->insert(x, y);
+            if (tiling) {
+                const int32_t xl = (int32_t)((float)W * 0.05f), xr = (int32_t)W - xl;
+                const int32_t yb = (int32_t)((float)H * 0.05f), yt = (int32_t)H - yb;
+                if (x < xl) grid->insert(x + (int32_t)W, y);
+                else if (x > xr) grid->insert(x - (int32_t)W, y);
+                if (y < yb) grid->insert(x, y + (int32_t)H);
+                else if (y > yt) grid->insert(x, y - (int32_t)H);
+            }
+            j->resolved.push_back(prev[q]);
+            j->rscore.push_back(prev_s[q]);
+        }
+    }
+    bool fanned = false;
+    std::mutex unresolved_mu;
+    const int P = (int)j->prm.backtrack_stages;
+    for (j->t = 0; j->t <= P; ++j->t) {
+        if (!open_stage(j)) return false;
+        const uint64_t redo_count = j->redo;
+        const unsigned n_workers = redo_count < 1000 ? 1u : std::max(1u, max_workers);
+        if (!fanned && n_workers > 1) {
+            fanned = true;
+            const uint32_t cell = (uint32_t)std::sqrt((double)(k * W * H / (uint32_t)redo_count)) * 2 + 1;
+            std::unique_ptr<TreeGridR> ng(new TreeGridR(tw, th, cell, offx, offy));
+            for (GridCell& c : grid->cells)
+                for (auto& bk : c.b)
+                    for (uint64_t pt : bk) ng->insert((int32_t)(uint32_t)(pt >> 32), (int32_t)(uint32_t)pt);
+            grid = std::move(ng);
+        }
+        const uint64_t S = j->S;
+        std::atomic<uint64_t> processed{0};
+        std::vector<std::vector<std::pair<uint32_t, float>>> lists(n_workers);
+        std::vector<double> ssum(n_workers, 0.0);
+        std::vector<uint64_t> scnt(n_workers, 0), sitems(n_workers, 0);
+        std::vector<Scratch> scr(n_workers);
+        auto worker = [&](unsigned tid) {
+            Scratch& s = scr[tid];
+            for (;;) {
+                const uint64_t i = processed.fetch_add(1, std::memory_order_relaxed);
+                if (i >= S) break;
+                const uint64_t loop_seed = j->pss + i;
+                uint32_t flat;
+                bool update_list;
+                if (i < redo_count) {
+                    update_list = false;
+                    flat = j->resolved[i + j->locked];
+                } else {
+                    update_list = true;
+                    std::lock_guard<std::mutex> lk(unresolved_mu);
+                    if (!pick_unresolved(j, loop_seed, flat)) break;
+                }
+                const int32_t x = (int32_t)(flat % W), y = (int32_t)(flat / W);
+                grid->knn((uint32_t)x, (uint32_t)y, k, s.nx, s.ny);
+                if (!s.nx.empty()) {
+                    weights_of(j, s, x, y);
+                    find_candidates(j, s, x, y, loop_seed + 1, LiveRead{j});
+                    const size_t kf = s.nx.size();
+                    s.pat.resize(kf * 4);
+                    for (size_t q = 0; q < kf; ++q) {
+                        int32_t nx = s.nx[q], ny = s.ny[q];
+                        if (tiling) {
+                            nx = modulo(nx, (int32_t)W);
+                            ny = modulo(ny, (int32_t)H);
+                        }
+                        const bool in = nx >= 0 && ny >= 0 && nx < (int32_t)W && ny < (int32_t)H;
+                        uint32_t c = 0xff000000u;
+                        if (in) c = ld32(reinterpret_cast<const uint32_t*>(&j->color[((size_t)ny * W + nx) * 4]));
+                        memcpy(&s.pat[q * 4], &c, 4);
+                    }
+                    if (j->guided) pattern_of(*j->tgl, s, tiling, s.gpat);
+                    size_t best;
+                    float lowest;
+                    find_best_match(j, s, best, lowest);
+                    const Cand& c = s.cands[best];
+                    live_update(j, *grid, lists[tid], flat, (uint32_t)c.x, (uint32_t)c.y, c.map, c.patch, c.pmap,
+                                update_list, lowest, tiling);
+                    if (lowest >= 0.0f) {
+                        ssum[tid] += lowest;
+                        scnt[tid] += 1;
+                    }
+                } else {
+                    /* resolve_at_random (ms.rs:447-475) */
+                    uint32_t rmap, rx, ry;
+                    random_source(j->pss, j->maps, rmap, rx, ry);
+                    live_update(j, *grid, lists[tid], flat, rx, ry, rmap, flat, rmap, true, 0.0f, false);
+                }
+                sitems[tid] += 1;
+            }
+        };
+        if (n_workers == 1) {
+            worker(0);
+        } else {
+            std::vector<std::thread> ths;
+            for (unsigned t = 0; t < n_workers; ++t) ths.emplace_back(worker, t);
+            for (auto& t : ths) t.join();
+        }
+        for (const Scratch& s : scr)
+            if (s.error) return set_err(j, "map id outside the example list");
+        /* the per-worker lists join the resolved list in worker order */
+        for (unsigned t = 0; t < n_workers; ++t) {
+            for (const auto& e : lists[t]) {
+                j->mask[e.first] = 1;
+                j->resolved.push_back(e.first);
+                j->rscore.push_back(e.second);
+            }
+            j->ssum[j->t] += ssum[t];
+            j->scnt[j->t] += scnt[t];
+            j->sitems[j->t] += sitems[t];
+        }
+        j->stage_open = false;
+    }
+    j->finished = true;
+    return true;
+}
+
+}  // namespace
 
 Img to_img(const orc_image& im) {
     Img r;
@@ -1206,6 +1539,13 @@ int orc_job_run(orc_job* j, const orc_sched* s, uint32_t n_threads) {
     j->sched = *s;
     j->threads = n_threads ? n_threads : 1;
     if (!advance(j, UINT32_MAX, 0)) return -1;
+    return 0;
+}
+
+int orc_job_run_reference_threads(orc_job* j, uint32_t n_threads) {
+    if (!j->err.empty()) return -1;
+    if (j->initialized) return set_err(j, "the job has already started"), -1;
+    if (!run_reference_threads(j, n_threads ? n_threads : 1)) return -1;
     return 0;
 }
 

@@ -91,6 +91,15 @@ const char* orc_job_error(orc_job* j);
 
 /* Runs Session::run (random init, then resolve) to completion. */
 int orc_job_run(orc_job* j, const orc_sched* s, uint32_t n_threads);
+/* Runs Session::run with the reference's own threading (ms.rs:815-1036),
+ * for timing: per stage one worker while fewer than 1000 items are to be
+ * redone, else n_threads workers on an atomic item counter, each item
+ * reading and writing the shared maps at once, k nearest from the TreeGrid
+ * (ms.rs:1313-1530; per-cell exact search, fan-out cells at the first
+ * parallel stage). With more than one worker the result depends on thread
+ * timing, as the reference's does; with one it equals orc_job_run with one
+ * pixel per batch. */
+int orc_job_run_reference_threads(orc_job* j, uint32_t n_threads);
 /* Runs until the given stage index (0 = first stage, p_stage = P) and
  * work item, leaving the state frozen at that point. */
 int orc_job_run_until(orc_job* j, const orc_sched* s, uint32_t n_threads, uint32_t stage_t, uint64_t item);

@@ -65,6 +65,7 @@ def lib():
         L.orc_job_error.restype = C.c_char_p
         L.orc_job_run.argtypes = [C.c_void_p, P(Sched), C.c_uint32]
         L.orc_job_run_until.argtypes = [C.c_void_p, P(Sched), C.c_uint32, C.c_uint32, C.c_uint64]
+        L.orc_job_run_reference_threads.argtypes = [C.c_void_p, C.c_uint32]
         L.orc_job_get.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_job_resolved.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]
         L.orc_job_resolved.restype = C.c_uint64
@@ -268,6 +269,15 @@ class Job:
     def run(self, threads=1, **sched_kw):
         s = sched(**sched_kw) if sched_kw else sched(**SEQUENTIAL)
         self._check(lib().orc_job_run(self._h, C.byref(s), threads))
+        return self
+
+    def run_reference_threads(self, threads=1):
+        """Session::run with the reference's own threading (timing mode):
+        serial while fewer than 1000 items are to be redone, then `threads`
+        workers with immediate updates and the TreeGrid kNN. With threads > 1
+        the result depends on timing; with 1 it equals run() one pixel per
+        batch."""
+        self._check(lib().orc_job_run_reference_threads(self._h, int(threads)))
         return self
 
     def run_until(self, stage_t, item, threads=1, **sched_kw):
