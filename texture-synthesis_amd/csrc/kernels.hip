@@ -407,6 +407,12 @@ __device__ __forceinline__ void spiral_tile(int t, int& ox, int& oy) {
     }
 }
 
+// Tiles each lane reads per round trip of the ring search. One (64 tiles a
+// round trip) keeps the search's registers few: under the engine's 80-VGPR
+// bound it spills 28 VGPRs, against 48 at 4 and 35 at 2; the 256-job mix ran 1.03, 0.98-1.00
+// and 0.96 s at 4, 2 and 1, the plain 1024^2 step the same within 1%.
+constexpr int kRingU = 1;
+
 // Ring search over the 8x8-tile resolved mask: 64 tiles per step in spiral
 // order, one tile per lane, the set bits of each tile taken lane-parallel.
 // Stops when k collected points lie closer than any pixel outside the
@@ -425,14 +431,14 @@ __device__ __forceinline__ void knn_rings(const KnnView& a, WaveLds* L, int x, i
     const int rmax = max(max(tx - g_x0, g_x1 - tx), max(ty - g_y0, g_y1 - ty));
     const int t_end = (2 * rmax + 1) * (2 * rmax + 1);
     iters = 0;
-    for (int t0 = 0; t0 < t_end; t0 += 4 * kWave) {
+    for (int t0 = 0; t0 < t_end; t0 += kRingU * kWave) {
         ++iters;
-        // 256 tiles per memory round trip, then their set bits lane-parallel
-        uint64_t word[4], nword[4];
-        uint32_t tmin[4];
-        int gx[4], gy[4];
+        // kRingU * 64 tiles per memory round trip, then their set bits lane-parallel
+        uint64_t word[kRingU], nword[kRingU];
+        uint32_t tmin[kRingU];
+        int gx[kRingU], gy[kRingU];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kRingU; ++u) {
             const int t = t0 + u * kWave + lane;
             word[u] = nword[u] = 0;
             tmin[u] = 0xffffffffu;
@@ -455,11 +461,11 @@ __device__ __forceinline__ void knn_rings(const KnnView& a, WaveLds* L, int x, i
         }
         if (a.mask_new) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < kRingU; ++u)
                 if (nword[u] && tmin[u] < a.n_at) word[u] |= filter_new(a, gx[u], gy[u], nword[u]);
         }
 #pragma unroll 1
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kRingU; ++u) {
             uint64_t rest = word[u];
             while (__ballot(rest != 0)) {
                 const bool have = rest != 0;
@@ -473,7 +479,7 @@ __device__ __forceinline__ void knn_rings(const KnnView& a, WaveLds* L, int x, i
             }
         }
         // rings fully visited after these tiles
-        const int rc = (isqrt_small(min(t0 + 4 * kWave, t_end)) - 1) >> 1;
+        const int rc = (isqrt_small(min(t0 + kRingU * kWave, t_end)) - 1) >> 1;
         if (rc >= rmax) break;
         if (s.count >= k && rc >= 0) {
             int64_t D = INT64_MAX, e;
