@@ -143,7 +143,10 @@ __device__ __forceinline__ uint64_t wave_sort64(uint64_t v, int lane) {
 // distance T is found by bisection (ballot counts), the keys with d^2 <= T
 // (k plus ties) are compacted and register-sorted; a tie group that pushes
 // them past 64 takes the full LDS sort instead.
-__device__ __noinline__ void sel_compact(uint64_t* keys, Sel& s, uint32_t k, int lane) {
+// The selection state is passed and returned by value: taken by reference it
+// would live in scratch memory (its address escapes to this out-of-line call),
+// and every push in the search loops would read and write it there.
+__device__ __noinline__ Sel sel_compact(uint64_t* keys, Sel s, uint32_t k, int lane) {
     const uint32_t cnt = s.count;
     if (cnt <= (uint32_t)kWave) {
         uint64_t v = (uint32_t)lane < cnt ? keys[lane] : ~0ull;
@@ -153,7 +156,7 @@ __device__ __noinline__ void sel_compact(uint64_t* keys, Sel& s, uint32_t k, int
         wave_sync();
         s.count = cnt < k ? cnt : k;
         s.kth = s.count == k ? keys[k - 1] : ~0ull;
-        return;
+        return s;
     }
     uint64_t v[kKeyCap / kWave];
     uint32_t hi = 0;
@@ -199,6 +202,7 @@ __device__ __noinline__ void sel_compact(uint64_t* keys, Sel& s, uint32_t k, int
     }
     s.count = k;
     s.kth = keys[k - 1];
+    return s;
 }
 
 __device__ __forceinline__ void sel_push(uint64_t* keys, Sel& s, uint32_t k, bool valid, uint64_t key, int lane) {
@@ -208,7 +212,7 @@ __device__ __forceinline__ void sel_push(uint64_t* keys, Sel& s, uint32_t k, boo
     s.count += (uint32_t)__popcll(m);
     if (s.count > kKeyCap - kWave) {
         wave_sync();
-        sel_compact(keys, s, k, lane);
+        s = sel_compact(keys, s, k, lane);
     }
 }
 
@@ -554,7 +558,7 @@ __device__ __forceinline__ uint32_t knn_search(const KnnView& v, WaveLds* L, int
         else knn_rings(v, L, x, y, k, lane, s, iters);
         buffered = s.count;
         wave_sync();
-        sel_compact(L->keys, s, k, lane);
+        s = sel_compact(L->keys, s, k, lane);
         found = s.count;
         if (lane < (int)found) L->nd[lane] = key_to_packed(L->keys[lane]);
     }
