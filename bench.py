@@ -13,6 +13,8 @@ k_resolve_persist for single-wave items, k_resolve for teams) from the
 launches' device-clock spans in the timed steps, and the CPU oracle timed on
 the host's cores.
 
+Beside the main line: "guided" (4 x 300^2 examples with guides -> 1024^2)
+and "plain_2048" (the plain path at 2048^2, with the job's device bytes).
 Two more legs report in the same line:
 * "batched": 256 independent syntheses (examples uniform in {64..512}^2,
   outputs uniform in {128..1024}^2, k in {20, 50}, m = 50, seeds 4000+j)
@@ -207,6 +209,32 @@ def batched_leg(ts, synth, n, sequential=True):
     }
 
 
+def plain_2048_leg(ts, synth, steps):
+    """The plain path one size up: 512^2 example (seed 512) -> 2048^2, default
+    parameters, seed 0, outputs resident; and the job's device bytes after
+    its run, the stage-ahead sets included."""
+    ex = synth.example(512, 512, 512)
+
+    def build():
+        return ts.Session.builder().add_example(ex).seed(0).output_size(ts.Dims.square(2048)).build()
+
+    s = build()
+    s.run_resident()
+    fp = s.footprint()
+    s.close()
+    lat = []
+    for _ in range(steps):
+        s = build()
+        t0 = time.perf_counter()
+        s.run_resident()
+        lat.append(time.perf_counter() - t0)
+        s.close()
+    return {"workload": "plain: 512x512 sinusoid+noise example (seed 512) -> 2048x2048, defaults, seed 0",
+            "value": round(2048 * 2048 * steps / sum(lat) / 1e6, 3), "unit": "Mpix/s",
+            "ms_per_step": round(1e3 * sum(lat) / steps, 3), "steps": steps,
+            "device_bytes_after_run": fp}
+
+
 def guided_leg(ts, synth, steps):
     """Multi-example guided synthesis: 4 examples of 300^2 with blob guides,
     a 1024^2 blob target guide, guide_alpha 0.8, random_init 10, k=50, seed 211."""
@@ -377,6 +405,7 @@ def main():
     ap.add_argument("--no-sequential", dest="sequential", action="store_false",
                     help="batched leg without the one-after-another runs (for profiling the engine)")
     ap.add_argument("--guided", type=int, default=3, help="runs of the guided leg (0 = skip)")
+    ap.add_argument("--plain2048", type=int, default=2, help="runs of the 2048^2 plain leg (0 = skip)")
     ap.add_argument("--serve-requests", type=int, default=2000, help="requests in the serving leg (0 = skip)")
     ap.add_argument("--serve-tight-requests", type=int, default=300,
                     help="requests of a second serving run under a tight memory budget (0 = skip)")
@@ -486,6 +515,8 @@ def main():
     }
     if args.guided:
         out["guided"] = guided_leg(ts, synth, args.guided)
+    if args.plain2048:
+        out["plain_2048"] = plain_2048_leg(ts, synth, args.plain2048)
     if args.batched:
         out["batched"] = batched_leg(ts, synth, args.batched, args.sequential)
         out["batched"]["vs_plain"] = round(out["batched"]["value"] / value, 2)

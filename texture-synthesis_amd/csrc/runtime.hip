@@ -2286,6 +2286,18 @@ int batch_round(tsg_batch* b) {
             }
             std::stable_sort(order[kd].begin(), order[kd].end(), [&](size_t x, size_t y) { return xcc_of[x] < xcc_of[y]; });
         }
+        // A runtime failure from here on (the block, the launches, the read
+        // back) leaves the launch's jobs undefined: some of their planned
+        // phases may have run. They leave with the failure, each with its
+        // own status, and the batch goes on with the others.
+        auto round_ok = [&](hipError_t e, const char* what) {
+            if (e == hipSuccess) return true;
+            const int c = fail(e == hipErrorOutOfMemory ? TSG_ERR_OUT_OF_MEMORY : TSG_ERR_DEVICE,
+                               std::string(what) + ": " + hipGetErrorString(e));
+            for (size_t q = b->active.size(); q-- > 0;)
+                if (std::find(runs.begin(), runs.end(), q) != runs.end()) eng_drop(b, q, c);
+            return false;
+        };
         // sizes of the launch block
         size_t n_jobs = runs.size(), n_ph = 0, n_args = 0, n_at = 0;
         for (size_t q : runs) {
@@ -2301,12 +2313,13 @@ int batch_round(tsg_batch* b) {
             // grown between launches (nothing reads the old block any more)
             const size_t cap = std::max(bytes, 2 * b->h_cap);
             char* h = nullptr;
-            HIP_TRY(hipHostMalloc((void**)&h, cap));
+            if (!round_ok(hipHostMalloc((void**)&h, cap), "hipHostMalloc(launch block)")) return TSG_OK;
             DBuf<char> d;
             const hipError_t e = d.alloc(cap);
             if (e != hipSuccess) {
                 (void)hipHostFree(h);
-                HIP_TRY(e);
+                round_ok(e, "launch block on the device");
+                return TSG_OK;
             }
             if (b->h_blk) (void)hipHostFree(b->h_blk);
             b->h_blk = h;
@@ -2366,11 +2379,12 @@ int batch_round(tsg_batch* b) {
             count[kd] = jx - first[kd];
             for (uint32_t y = 0; y <= 8; ++y) C.job0[y] += (uint32_t)first[kd];
         }
-        HIP_TRY(hipMemcpyAsync(D, H, o_nat + n_at * 4, hipMemcpyHostToDevice, s));
+        if (!round_ok(hipMemcpyAsync(D, H, o_nat + n_at * 4, hipMemcpyHostToDevice, s), "launch block upload"))
+            return TSG_OK;
         lap(8);
         for (int kd = 0; kd < 2; ++kd) {
             if (!count[kd]) continue;
-            if (!b->wgs[kd]) return fail(TSG_ERR_DEVICE, "engine occupancy query failed");
+            if (!b->wgs[kd] && !round_ok(hipErrorInvalidValue, "engine occupancy query")) return TSG_OK;
             QEngineArgs ea{};
             ea.jobs = reinterpret_cast<const QJob*>(D + o_job);
             ea.dyn = reinterpret_cast<QDyn*>(D + o_dyn);
@@ -2380,13 +2394,17 @@ int batch_round(tsg_batch* b) {
             ea.idle_ticks = b->idle_ticks;
             ea.flags = spread[kd] ? kEngineSpread : 0u;
             const uint32_t wgs = b->wgs_cap ? std::min(b->wgs_cap, b->wgs[kd]) : b->wgs[kd];
-            HIP_TRY(launch_engine(ea, reinterpret_cast<const BatchArgs*>(D + o_args), wgs, kd == 1, s));
+            if (!round_ok(launch_engine(ea, reinterpret_cast<const BatchArgs*>(D + o_args), wgs, kd == 1, s),
+                          "launch_engine"))
+                return TSG_OK;
             b->launches += 1;
         }
         // control words and every job's cursor: each program must have run to its end
         std::vector<char> back(o_ph - o_ctl);
-        HIP_TRY(hipMemcpyAsync(back.data(), D + o_ctl, back.size(), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
+        if (!round_ok(hipMemcpyAsync(back.data(), D + o_ctl, back.size(), hipMemcpyDeviceToHost, s),
+                      "control words read back") ||
+            !round_ok(hipStreamSynchronize(s), "engine launch"))
+            return TSG_OK;
         lap(9);
         const QCtl* bc = reinterpret_cast<const QCtl*>(back.data());
         uint32_t err[2] = {0, 0};
