@@ -27,11 +27,11 @@ Two more legs report in the same line:
   elliptical holes covering 20-40% (the example resized to the output with
   CatmullRom); admission by device-memory footprint; p50/p99 latency
   (arrival to result on the host) and output Mpix/s. Requests starting
-  while fewer than 8 are in flight run alone with the whole device; as they
+  while fewer than 32 are in flight run alone with the whole device; as they
   pile up they share the work-queue engine (the line says how many took
-  which path). A third run at a higher rate ("serving_loaded") shows the
-  engine taking over, and a second at that rate under a tight budget shows
-  admission held back by memory. The requests' images are generated before the
+  which path). A third run at an overloading rate ("serving_loaded", 40/s)
+  shows the engine taking over, and a second at 30/s under a tight budget
+  shows admission held back by memory. The requests' images are generated before the
   clock starts.
 """
 import hashlib
@@ -413,7 +413,7 @@ def main():
     ap.add_argument("--serve-loaded-requests", type=int, default=400,
                     help="requests of a third serving run at --serve-loaded-qps, where requests pile up "
                          "and share the work-queue engine (0 = skip)")
-    ap.add_argument("--serve-loaded-qps", type=float, default=30.0)
+    ap.add_argument("--serve-loaded-qps", type=float, default=40.0)
     ap.add_argument("--serve-qps", type=float, default=18.0,
                     help="Poisson arrival rate; 18/s keeps the engine busy about 70%% of the time")
     ap.add_argument("--serve-budget-gib", type=float, default=4.0)
@@ -523,8 +523,7 @@ def main():
     if args.serve_requests:
         out["serving"] = serving_leg(ts, synth, args.serve_requests, args.serve_qps, args.serve_budget_gib)
     if args.serve_tight_requests:
-        out["serving_tight_budget"] = serving_leg(ts, synth, args.serve_tight_requests, args.serve_loaded_qps,
-                                                  args.serve_tight_gib)
+        out["serving_tight_budget"] = serving_leg(ts, synth, args.serve_tight_requests, 30.0, args.serve_tight_gib)
     if args.serve_loaded_requests:
         out["serving_loaded"] = serving_leg(ts, synth, args.serve_loaded_requests, args.serve_loaded_qps,
                                             args.serve_budget_gib)

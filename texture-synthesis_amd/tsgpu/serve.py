@@ -7,9 +7,9 @@ estimate fit the memory budget and an engine slot is free: it is then built
 ``Batch``, whose rounds run every admitted request's planned work in one
 launch of the work-queue engine. Requests that arrive while others run join
 at the next round, and a finished request frees its memory and slot for the
-next in line. Under light load (fewer than ALONE_BELOW requests in flight) a
-request starting runs alone with the whole device instead, which shortens
-its chain of small batches; the engine takes over as requests pile up. A request whose estimate alone exceeds the budget is rejected
+next in line. Unless ALONE_BELOW or more requests are in flight, a request
+starting runs alone with the whole device instead, which shortens its chain
+of small batches; the engine takes over as requests pile up. A request whose estimate alone exceeds the budget is rejected
 (AdmissionRejected) rather than held; a request can be cancelled; a request
 that fails (invalid, out of memory) is reported without disturbing others.
 
@@ -32,7 +32,10 @@ CANCELLED = -23
 # resolve items per request per engine round (0: no bound)
 QUANTUM = 0
 # a request that starts while fewer than this many are in flight runs alone
-ALONE_BELOW = 8
+# (on the 128^2..2048^2 serving mix: p99 215 against 219 ms at 18 requests/s,
+# 320 against 535 ms at 30/s, with 32 rather than 8; the engine pays off only
+# once requests pile up)
+ALONE_BELOW = 32
 
 
 @dataclass
