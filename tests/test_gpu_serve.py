@@ -35,13 +35,16 @@ def same(a, b):
             and np.array_equal(a.resolved_score.view(np.uint32), b.resolved_score.view(np.uint32)))
 
 
-@pytest.mark.parametrize("budget_mb,max_active", [(4096, 64), (0, 3)])
-def test_serving_results_equal_alone(ctx, budget_mb, max_active):
+@pytest.mark.parametrize("budget_mb,max_active,alone_below",
+                         [(4096, 64, serve.ALONE_BELOW), (0, 3, serve.ALONE_BELOW), (4096, 64, 0), (0, 3, 0)])
+def test_serving_results_equal_alone(ctx, budget_mb, max_active, alone_below):
+    # alone_below 0: every request shares the work-queue engine; the default
+    # runs these few requests alone on the single-job path
     reqs = mix()
     alone = [request(*r).build().run() for r in reqs]
     # 0: a budget that just fits the largest request, so admissions wait for memory
     budget = budget_mb << 20 if budget_mb else max(request(*r).footprint_estimate() for r in reqs)
-    srv = serve.Server(ctx, hbm_budget=budget, max_active=max_active, rounds_per_pump=5)
+    srv = serve.Server(ctx, hbm_budget=budget, max_active=max_active, rounds_per_pump=5, alone_below=alone_below)
     got = {}
     pending = list(enumerate(reqs))
     # arrivals interleaved with engine rounds: two requests join per pump
@@ -62,6 +65,8 @@ def test_serving_results_equal_alone(ctx, budget_mb, max_active):
     assert srv.in_use == 0 and srv.rejected == 0
     if not budget_mb:
         assert srv.memory_holds > 0
+    st = srv.batch.stats()
+    assert (st["alone"] == 0 and st["launches"] > 0) if alone_below == 0 else st["alone"] == len(reqs)
     srv.close()
 
 
