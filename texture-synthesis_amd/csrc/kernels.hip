@@ -47,6 +47,23 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A copy of a record that every lane reads at the same address (job
+// programs, phases, argument blocks), each 32-bit word moved to a scalar
+// register as it arrives. After an acquire fence the compiler cannot treat
+// such records as unchanged since the launch, so it reads them with vector
+// loads; held as loaded, their fields took some 30 VGPRs plus spills through
+// the whole resolve loop, while as scalars they cost no VGPRs.
+template <class T>
+__device__ __forceinline__ T ld_uniform(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "record of whole words");
+    T v;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(p);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&v);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = (uint32_t)__builtin_amdgcn_readfirstlane((int)src[i]);
+    return v;
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt(int lane) { return (1ull << lane) - 1ull; }
 
 // Canonical neighbour order: squared distance, then dy, then dx, ascending.
@@ -571,17 +588,62 @@ __device__ __forceinline__ bool coord_ok(const MapDesc& md, int x, int y) {
     return md.smask == nullptr || md.smask[(uint32_t)y * md.w + (uint32_t)x] != 0;
 }
 
+// Byte c of v times 2^SH: one SDWA shift. The compiler turns byte 0 into a
+// shift and a mask, so that one is written out (a pure VALU instruction).
+template <int c, int SH>
+__device__ __forceinline__ uint32_t byte_scaled(uint32_t v) {
+    if constexpr (c == 0) {
+        uint32_t r;
+        asm("v_lshlrev_b32_sdwa %0, %2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_0"
+            : "=v"(r)
+            : "v"(v), "I"(SH));
+        return r;
+    } else {
+        return ((v >> (8 * c)) & 255u) << SH;
+    }
+}
+
+// Cost tables are interleaved C ways in LDS: entry d of copy s is word
+// d*C + s. Channel bytes scaled by the entry stride (4C bytes) give an
+// entry's byte offset from their absolute difference, one v_sad_u32 with
+// the copy's byte offset slot4 as the addend.
 template <int C>
-__device__ __forceinline__ float lut4(const float* lut, uint32_t q, uint32_t p, int slot) {
-    // four channel terms in R, G, B, A order into one accumulator
-    int d0 = abs((int)(q & 255u) - (int)(p & 255u));
-    int d1 = abs((int)((q >> 8) & 255u) - (int)((p >> 8) & 255u));
-    int d2 = abs((int)((q >> 16) & 255u) - (int)((p >> 16) & 255u));
-    int d3 = abs((int)(q >> 24) - (int)(p >> 24));
-    float s = lut[d0 * C + slot];
-    s = s + lut[d1 * C + slot];
-    s = s + lut[d2 * C + slot];
-    s = s + lut[d3 * C + slot];
+constexpr int lut_shift() {
+    static_assert(C == 8 || C == 4, "table copies");
+    return C == 8 ? 5 : 4;
+}
+__device__ __forceinline__ float lut_sad(const float* lut, uint32_t a, uint32_t b, uint32_t slot4) {
+    const uint32_t off = (a > b ? a - b : b - a) + slot4;
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(lut) + off);
+}
+
+// The four channel bytes of v, scaled for lut_sad.
+template <int C>
+__device__ __forceinline__ uint4 bytes_scaled(uint32_t v) {
+    constexpr int SH = lut_shift<C>();
+    return make_uint4(byte_scaled<0, SH>(v), byte_scaled<1, SH>(v), byte_scaled<2, SH>(v), byte_scaled<3, SH>(v));
+}
+
+// s plus the four channel terms |q_c - p_c| of table `lut`, added in R, G, B,
+// A order; q is given scaled (bytes_scaled).
+template <int C>
+__device__ __forceinline__ float lut4_add(float s, const float* lut, const uint4& q, uint32_t p, uint32_t slot4) {
+    constexpr int SH = lut_shift<C>();
+    s = s + lut_sad(lut, q.x, byte_scaled<0, SH>(p), slot4);
+    s = s + lut_sad(lut, q.y, byte_scaled<1, SH>(p), slot4);
+    s = s + lut_sad(lut, q.z, byte_scaled<2, SH>(p), slot4);
+    s = s + lut_sad(lut, q.w, byte_scaled<3, SH>(p), slot4);
+    return s;
+}
+
+// The four channel terms alone (the first one starts the sum).
+template <int C>
+__device__ __forceinline__ float lut4(const float* lut, const uint4& q, uint32_t p, uint32_t slot4) {
+    constexpr int SH = lut_shift<C>();
+    float s = lut_sad(lut, q.x, byte_scaled<0, SH>(p), slot4);
+    s = s + lut_sad(lut, q.y, byte_scaled<1, SH>(p), slot4);
+    s = s + lut_sad(lut, q.z, byte_scaled<2, SH>(p), slot4);
+    s = s + lut_sad(lut, q.w, byte_scaled<3, SH>(p), slot4);
     return s;
 }
 
@@ -607,37 +669,60 @@ __device__ __forceinline__ void cand_get(const BatchArgs& a, const WaveLds* L, u
 // Gather of a candidate's pattern pixel for neighbour j (mirrored for
 // random candidates, ms.rs:534/588): colour and, when guided, guide pixel;
 // out-of-bounds pixels are [0,0,0,255] (get_color_of_neighbor, ms.rs:1122-1149).
+// The descriptor's fields are read together, ahead of the bounds test, so
+// that their loads are not chained behind one another.
 template <bool GUIDED>
-__device__ __forceinline__ void gather_px(const MapDesc* maps, const WaveLds* L, uint32_t xy, uint32_t ms, uint32_t j,
+__device__ __forceinline__ void gather_md(const MapDesc& md, uint32_t xy, uint32_t ms, int dx, int dy,
                                           uint32_t& p, uint32_t& gp) {
-    const MapDesc& md = maps[ms & 0x7fffffffu];
-    const int sgn = (ms >> 31) ? -1 : 1;
-    const int32_t o = L->nd[j];
-    const int X2 = (int)(xy & 0xffffu) + sgn * packed_dx(o), Y2 = (int)(xy >> 16) + sgn * packed_dy(o);
-    const bool in = (uint32_t)X2 < md.w && (uint32_t)Y2 < md.h;
-    p = in ? md.px[(uint32_t)Y2 * md.w + (uint32_t)X2] : kOutsidePx;
+    const uint32_t* px = md.px;
+    const uint32_t w = md.w, h = md.h;
+    const bool mir = (ms >> 31) != 0;
+    const int X2 = (int)(xy & 0xffffu) + (mir ? -dx : dx), Y2 = (int)(xy >> 16) + (mir ? -dy : dy);
+    const bool in = (uint32_t)X2 < w && (uint32_t)Y2 < h;
+    p = in ? px[(uint32_t)Y2 * w + (uint32_t)X2] : kOutsidePx;
     gp = kOutsidePx;
     if (GUIDED) {
-        const bool gin = (uint32_t)X2 < md.gw && (uint32_t)Y2 < md.gh;
-        gp = gin ? md.gpx[(uint32_t)Y2 * md.gw + (uint32_t)X2] : kOutsidePx;
+        const uint32_t* gpx = md.gpx;
+        const uint32_t gw = md.gw, gh = md.gh;
+        const bool gin = (uint32_t)X2 < gw && (uint32_t)Y2 < gh;
+        gp = gin ? gpx[(uint32_t)Y2 * gw + (uint32_t)X2] : kOutsidePx;
     }
 }
+template <bool GUIDED>
+__device__ __forceinline__ void gather_at(const MapDesc* maps, uint32_t xy, uint32_t ms, int dx, int dy,
+                                          uint32_t& p, uint32_t& gp) {
+    gather_md<GUIDED>(maps[ms & 0x7fffffffu], xy, ms, dx, dy, p, gp);
+}
 
-// Cost term of neighbour j: (four colour LUT terms, then four guide LUT
+// Target side of neighbour j, the same for every candidate: its pattern and
+// guide pixels scaled for lut_sad, its offset and its weight.
+struct NbrRow {
+    uint4 q, g;
+    int dx, dy;
+    float w;
+};
+template <bool GUIDED>
+__device__ __forceinline__ NbrRow nbr_row(const WaveLds* L, uint32_t j) {
+    constexpr int C = lut_copies<GUIDED>();
+    NbrRow r;
+    r.q = bytes_scaled<C>(L->pat[j]);
+    r.g = GUIDED ? bytes_scaled<C>(L->gpat[j]) : make_uint4(0, 0, 0, 0);
+    const int32_t o = L->nd[j];
+    r.dx = packed_dx(o);
+    r.dy = packed_dy(o);
+    r.w = L->w[j];
+    return r;
+}
+
+// Cost term of a neighbour: (four colour LUT terms, then four guide LUT
 // terms, summed in order) times w_j, as one rounded product.
 template <bool GUIDED>
-__device__ __forceinline__ float term_of(const WaveLds* L, const float* lut, const float* glut, int slot, uint32_t j,
+__device__ __forceinline__ float term_at(const NbrRow& r, const float* lut, const float* glut, uint32_t slot4,
                                          uint32_t p, uint32_t gp) {
     constexpr int C = lut_copies<GUIDED>();
-    float s = lut4<C>(lut, L->pat[j], p, slot);
-    if (GUIDED) {
-        const uint32_t q = L->gpat[j];
-        s = s + glut[abs((int)(q & 255u) - (int)(gp & 255u)) * lut_copies<GUIDED>() + slot];
-        s = s + glut[abs((int)((q >> 8) & 255u) - (int)((gp >> 8) & 255u)) * lut_copies<GUIDED>() + slot];
-        s = s + glut[abs((int)((q >> 16) & 255u) - (int)((gp >> 16) & 255u)) * lut_copies<GUIDED>() + slot];
-        s = s + glut[abs((int)(q >> 24) - (int)(gp >> 24)) * lut_copies<GUIDED>() + slot];
-    }
-    return __fmul_rn(s, L->w[j]);
+    float s = lut4<C>(lut, r.q, p, slot4);
+    if (GUIDED) s = lut4_add<C>(s, glut, r.g, gp, slot4);
+    return __fmul_rn(s, r.w);
 }
 
 struct Best {
@@ -669,8 +754,12 @@ __device__ __forceinline__ void prune_bound(const unsigned long long* tb, const 
 }
 
 template <bool GUIDED>
-__device__ __forceinline__ void eval_group(const MapDesc* maps, WaveLds* L, const float* lut, const float* glut, int slot, int lane,
-                           uint32_t kf, uint32_t n_act, Best& best, unsigned long long* tb) {
+__device__ __forceinline__ void eval_group(const MapDesc* maps, uint32_t n_maps, WaveLds* L, const float* lut, const float* glut,
+                                           int slot, int lane, uint32_t kf, uint32_t n_act, Best& best,
+                                           unsigned long long* tb) {
+    const uint32_t slot4 = (uint32_t)slot * 4u;
+    MapDesc m0{};
+    if (n_maps == 1) m0 = ld_uniform(maps);
     if (lane < (int)n_act) L->act_sum[lane] = 0.0f;
     wave_sync();
     float PB;
@@ -685,19 +774,33 @@ __device__ __forceinline__ void eval_group(const MapDesc* maps, WaveLds* L, cons
         const uint32_t jc = 1u << sh;
         const uint32_t jn = min(jc, kf - j0);
         const uint32_t total = n_act << sh;
-        // all gathers of the chunk first (at most kTermCap / 64 per lane), then the LUT sums
+        // jc divides the wave, so a lane's neighbour is the same in every
+        // slice u of the chunk: its row is read once
+        const uint32_t jj = (uint32_t)lane & (jc - 1);
+        const bool jin = jj < jn;
+        const NbrRow row = nbr_row<GUIDED>(L, j0 + (jin ? jj : 0u));
+        // all gathers of the chunk first (at most kTermCap / 64 per lane), then
+        // the LUT sums; a single example's descriptor is held in scalars
         uint32_t px[kTermCap / kWave], gpx[kTermCap / kWave];
+        if (n_maps == 1) {
 #pragma unroll
-        for (int u = 0; u < kTermCap / kWave; ++u) {
-            const uint32_t l = (uint32_t)(u * kWave + lane), ai = l >> sh, jj = l & (jc - 1);
-            px[u] = gpx[u] = 0;
-            if (l < total && ai < n_act && jj < jn)
-                gather_px<GUIDED>(maps, L, L->act_xy[ai], L->act_ms[ai], j0 + jj, px[u], gpx[u]);
+            for (int u = 0; u < kTermCap / kWave; ++u) {
+                const uint32_t l = (uint32_t)(u * kWave + lane), ai = l >> sh;
+                px[u] = gpx[u] = 0;
+                if (l < total && jin) gather_md<GUIDED>(m0, L->act_xy[ai], L->act_ms[ai], row.dx, row.dy, px[u], gpx[u]);
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < kTermCap / kWave; ++u) {
+                const uint32_t l = (uint32_t)(u * kWave + lane), ai = l >> sh;
+                px[u] = gpx[u] = 0;
+                if (l < total && jin) gather_at<GUIDED>(maps, L->act_xy[ai], L->act_ms[ai], row.dx, row.dy, px[u], gpx[u]);
+            }
         }
 #pragma unroll
         for (int u = 0; u < kTermCap / kWave; ++u) {
-            const uint32_t l = (uint32_t)(u * kWave + lane), ai = l >> sh, jj = l & (jc - 1);
-            if (l < total && ai < n_act && jj < jn) L->terms[l] = term_of<GUIDED>(L, lut, glut, slot, j0 + jj, px[u], gpx[u]);
+            const uint32_t l = (uint32_t)(u * kWave + lane);
+            if (l < total && jin) L->terms[l] = term_at<GUIDED>(row, lut, glut, slot4, px[u], gpx[u]);
         }
         wave_sync();
         const bool act = lane < (int)n_act;
@@ -959,6 +1062,7 @@ template <bool GUIDED>
 __device__ __forceinline__ void item_exact(const BatchArgs& a, const MapDesc* maps, uint32_t item, uint32_t rel,
                                            uint32_t kf, uint32_t n_coh, uint32_t n_c, WaveLds* L, const float* lut,
                                            const float* glut, int slot, int lane, float& best, uint32_t& bidx) {
+    const uint32_t slot4 = (uint32_t)slot * 4u;
     for (uint32_t base = 0; base < n_c; base += kWave) {
         const uint32_t c = base + lane;
         const bool v = c < n_c;
@@ -991,15 +1095,12 @@ __device__ __forceinline__ void item_exact(const BatchArgs& a, const MapDesc* ma
                 const bool in = (uint32_t)X2 < md.w && (uint32_t)Y2 < md.h;
                 const uint32_t idx = (uint32_t)Y2 * md.w + (uint32_t)X2;
                 const uint32_t p = in ? md.px[idx] : kOutsidePx;
-                float s = lut4<lut_copies<GUIDED>()>(lut, L->pat[j], p, slot);
+                constexpr int C = lut_copies<GUIDED>();
+                float s = lut4<C>(lut, bytes_scaled<C>(L->pat[j]), p, slot4);
                 if (GUIDED) {
                     const bool gin = (uint32_t)X2 < md.gw && (uint32_t)Y2 < md.gh;
                     const uint32_t gp = gin ? md.gpx[(uint32_t)Y2 * md.gw + (uint32_t)X2] : kOutsidePx;
-                    const uint32_t q = L->gpat[j];
-                    s = s + glut[abs((int)(q & 255u) - (int)(gp & 255u)) * lut_copies<GUIDED>() + slot];
-                    s = s + glut[abs((int)((q >> 8) & 255u) - (int)((gp >> 8) & 255u)) * lut_copies<GUIDED>() + slot];
-                    s = s + glut[abs((int)((q >> 16) & 255u) - (int)((gp >> 16) & 255u)) * lut_copies<GUIDED>() + slot];
-                    s = s + glut[abs((int)(q >> 24) - (int)(gp >> 24)) * lut_copies<GUIDED>() + slot];
+                    s = lut4_add<C>(s, glut, bytes_scaled<C>(L->gpat[j]), gp, slot4);
                 }
                 score = __fadd_rn(score, __fmul_rn(s, L->w[j]));
                 if (score == score) lastgood = score;
@@ -1247,7 +1348,7 @@ __device__ __forceinline__ uint4 resolve_team(const BatchArgs& a, const MapDesc*
                     L->act_idx[lane] = ci;
                 }
                 wave_sync();
-                eval_group<GUIDED>(maps, L, lut, glut, slot, lane, kf, g_hi - g_lo, bst, T > 1 ? &Lt->t_bound : nullptr);
+                eval_group<GUIDED>(maps, a.n_maps, L, lut, glut, slot, lane, kf, g_hi - g_lo, bst, T > 1 ? &Lt->t_bound : nullptr);
                 if (lead && first) phase_clock(a, item, 4, lane);
                 first = false;
                 g_lo = g_hi;
@@ -1956,23 +2057,6 @@ __device__ __forceinline__ uint32_t xcc_id() {
 // (device-scope acquire: the CU's L1 and the L2's non-coherent lines are
 // dropped).
 __device__ __forceinline__ void unit_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
-
-// A copy of a record that every lane reads at the same address (job
-// programs, phases, argument blocks), each 32-bit word moved to a scalar
-// register as it arrives. After an acquire fence the compiler cannot treat
-// such records as unchanged since the launch, so it reads them with vector
-// loads; held as loaded, their fields took some 30 VGPRs plus spills through
-// the whole resolve loop, while as scalars they cost no VGPRs.
-template <class T>
-__device__ __forceinline__ T ld_uniform(const T* p) {
-    static_assert(sizeof(T) % 4 == 0, "record of whole words");
-    T v;
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(p);
-    uint32_t* dst = reinterpret_cast<uint32_t*>(&v);
-#pragma unroll
-    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = (uint32_t)__builtin_amdgcn_readfirstlane((int)src[i]);
-    return v;
-}
 
 // Release to the workgroups of the same XCD: the wave waits until its stores
 // have completed to the XCD's L2, which every CU of the XCD reads through.
