@@ -64,6 +64,44 @@ __device__ __forceinline__ T ld_uniform(const T* p) {
     return v;
 }
 
+// Loads and stores of device memory through the global address space. For a
+// pointer read out of an argument record the compiler cannot tell device
+// memory from LDS and emits flat instructions; a flat access also counts as
+// an LDS access, so every later wait for an LDS read would wait for it too,
+// which serialises a lane's gathers behind its LDS reads. Only for pointers
+// to device buffers (never LDS).
+template <class T>
+struct same_type {
+    using type = T;
+};
+template <class T>
+__device__ __forceinline__ T gld(const T* p) {
+    return *(const __attribute__((address_space(1))) T*)p;
+}
+template <class T>
+__device__ __forceinline__ void gst(T* p, typename same_type<T>::type v) {
+    *(__attribute__((address_space(1))) T*)p = v;
+}
+// HIP's vector structs move as the native vectors they hold.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint2 gld(const uint2* p) {
+    const u32x2 v = *(const __attribute__((address_space(1))) u32x2*)p;
+    return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ uint4 gld(const uint4* p) {
+    const u32x4 v = *(const __attribute__((address_space(1))) u32x4*)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gst(uint4* p, uint4 v) {
+    u32x4 w;
+    w.x = v.x;
+    w.y = v.y;
+    w.z = v.z;
+    w.w = v.w;
+    *(__attribute__((address_space(1))) u32x4*)p = w;
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt(int lane) { return (1ull << lane) - 1ull; }
 
 // Canonical neighbour order: squared distance, then dy, then dx, ascending.
@@ -234,7 +272,7 @@ __device__ __forceinline__ void sel_push(uint64_t* keys, Sel& s, uint32_t k, boo
 }
 
 __device__ __forceinline__ bool px_set(const uint64_t* mask, uint32_t tiles_x, int X, int Y) {
-    uint64_t wd = mask[(uint32_t)(Y >> 3) * tiles_x + (uint32_t)(X >> 3)];
+    uint64_t wd = gld(mask + (uint32_t)(Y >> 3) * tiles_x + (uint32_t)(X >> 3));
     return (wd >> (((Y & 7) << 3) | (X & 7))) & 1ull;
 }
 
@@ -273,8 +311,8 @@ __device__ __forceinline__ int32_t* offtab_lds() {
 
 // The resolved bits of tile ti.
 __device__ __forceinline__ uint64_t mask_word(const KnnView& v, uint32_t ti) {
-    uint64_t w = v.mask[ti];
-    if (v.pend) w |= v.pend[ti];
+    uint64_t w = gld(v.mask + ti);
+    if (v.pend) w |= gld(v.pend + ti);
     return w;
 }
 
@@ -302,8 +340,8 @@ __device__ __forceinline__ uint32_t ghost_row(const KnnView& v, int Y, int xs) {
     const int c0 = xs >> 3, off = xs - c0 * 8, sh = (Y & 7) * 8;
     const uint32_t row = (uint32_t)(Y >> 3) * v.tiles_x;
     uint32_t b0 = 0, b1 = 0;
-    if (c0 >= 0 && c0 < (int)v.tiles_x) b0 = (uint32_t)(v.gmask[row + (uint32_t)c0] >> sh) & 0xffu;
-    if (c0 + 1 >= 0 && c0 + 1 < (int)v.tiles_x) b1 = (uint32_t)(v.gmask[row + (uint32_t)(c0 + 1)] >> sh) & 0xffu;
+    if (c0 >= 0 && c0 < (int)v.tiles_x) b0 = (uint32_t)(gld(v.gmask + row + (uint32_t)c0) >> sh) & 0xffu;
+    if (c0 + 1 >= 0 && c0 + 1 < (int)v.tiles_x) b1 = (uint32_t)(gld(v.gmask + row + (uint32_t)(c0 + 1)) >> sh) & 0xffu;
     return ((b0 | (b1 << 8)) >> off) & 0xffu;
 }
 
@@ -354,7 +392,7 @@ __device__ __forceinline__ uint64_t filter_new(const KnnView& v, int gx, int gy,
         }
 #pragma unroll
         for (int q = 0; q < 8; ++q)
-            r[q] = b[q] >= 0 ? v.rank[(uint32_t)(gy * 8 + (b[q] >> 3)) * v.W + (uint32_t)(gx * 8 + (b[q] & 7))]
+            r[q] = b[q] >= 0 ? gld(v.rank + (uint32_t)(gy * 8 + (b[q] >> 3)) * v.W + (uint32_t)(gx * 8 + (b[q] & 7)))
                              : 0xffffffffu;
 #pragma unroll
         for (int q = 0; q < 8; ++q)
@@ -371,7 +409,7 @@ __device__ __forceinline__ void knn_list(const KnnView& a, WaveLds* L, uint32_t 
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const uint32_t idx = base + u * kWave + lane;
-            f[u] = idx < n_res ? a.rlist[idx] : 0u;
+            f[u] = idx < n_res ? gld(a.rlist + idx) : 0u;
         }
 #pragma unroll 1
         for (int u = 0; u < 4; ++u) {
@@ -474,8 +512,8 @@ __device__ __forceinline__ void knn_rings(const KnnView& a, WaveLds* L, int x, i
                     word[u] = tile_points(a, gx[u], gy[u], in_grid);
                     if (a.mask_new && in_grid) {
                         const uint32_t ti = (uint32_t)gy[u] * a.tiles_x + (uint32_t)gx[u];
-                        nword[u] = a.mask_new[ti];
-                        tmin[u] = a.tile_min[ti];
+                        nword[u] = gld(a.mask_new + ti);
+                        tmin[u] = gld(a.tile_min + ti);
                     }
                 }
             }
@@ -532,12 +570,12 @@ __device__ __forceinline__ uint32_t knn_table(const KnnView& a, WaveLds* L, int 
             o[u] = 0;
             hit[u] = false;
             if (t < a.offtab_n) {
-                o[u] = (a.offc && base < kOffCache) ? a.offc[t] : a.offtab[t];
+                o[u] = (a.offc && base < kOffCache) ? a.offc[t] : gld(a.offtab + t);
                 const int X = x + packed_dx(o[u]), Y = y + packed_dy(o[u]);
                 const bool in = X >= 0 && Y >= 0 && X < (int)a.W && Y < (int)a.H;
                 hit[u] = in ? view_set(a, X, Y) : point_at(a, X, Y);
                 if (in && !hit[u] && a.mask_new && px_set(a.mask_new, a.tiles_x, X, Y))
-                    hit[u] = a.rank[(uint32_t)Y * a.W + (uint32_t)X] < a.n_at;
+                    hit[u] = gld(a.rank + (uint32_t)Y * a.W + (uint32_t)X) < a.n_at;
             }
         }
 #pragma unroll
@@ -585,7 +623,7 @@ __device__ __forceinline__ uint32_t knn_search(const KnnView& v, WaveLds* L, int
 
 __device__ __forceinline__ bool coord_ok(const MapDesc& md, int x, int y) {
     if (x < 0 || y < 0 || (uint32_t)x >= md.w || (uint32_t)y >= md.h) return false;
-    return md.smask == nullptr || md.smask[(uint32_t)y * md.w + (uint32_t)x] != 0;
+    return md.smask == nullptr || gld(md.smask + (uint32_t)y * md.w + (uint32_t)x) != 0;
 }
 
 // Byte c of v times 2^SH: one SDWA shift. The compiler turns byte 0 into a
@@ -660,7 +698,7 @@ __device__ __forceinline__ void cand_get(const BatchArgs& a, const WaveLds* L, u
         xy = e.x;
         ms = e.y;
     } else {
-        uint2 r = a.rcands[(size_t)(item - a.rc_base) * a.m + (ci - n_coh)];
+        uint2 r = gld(a.rcands + (size_t)(item - a.rc_base) * a.m + (ci - n_coh));
         xy = r.x;
         ms = r.y | 0x80000000u;
     }
@@ -679,13 +717,13 @@ __device__ __forceinline__ void gather_md(const MapDesc& md, uint32_t xy, uint32
     const bool mir = (ms >> 31) != 0;
     const int X2 = (int)(xy & 0xffffu) + (mir ? -dx : dx), Y2 = (int)(xy >> 16) + (mir ? -dy : dy);
     const bool in = (uint32_t)X2 < w && (uint32_t)Y2 < h;
-    p = in ? px[(uint32_t)Y2 * w + (uint32_t)X2] : kOutsidePx;
+    p = in ? gld(px + (uint32_t)Y2 * w + (uint32_t)X2) : kOutsidePx;
     gp = kOutsidePx;
     if (GUIDED) {
         const uint32_t* gpx = md.gpx;
         const uint32_t gw = md.gw, gh = md.gh;
         const bool gin = (uint32_t)X2 < gw && (uint32_t)Y2 < gh;
-        gp = gin ? gpx[(uint32_t)Y2 * gw + (uint32_t)X2] : kOutsidePx;
+        gp = gin ? gld(gpx + (uint32_t)Y2 * gw + (uint32_t)X2) : kOutsidePx;
     }
 }
 template <bool GUIDED>
@@ -906,8 +944,8 @@ __device__ __forceinline__ uint32_t item_knn(const BatchArgs& a, uint32_t item, 
         kf = a.ext_nbr_count[rel];
         if (lane < (int)kf) L->nd[lane] = a.ext_nbrs[(size_t)rel * k + lane];
     } else if (a.pre_nbrs && item >= a.pre_base && item < a.pre_end) {
-        kf = a.pre_cnt[item - a.pre_base];
-        if (lane < (int)kf) L->nd[lane] = a.pre_nbrs[(size_t)(item - a.pre_base) * k + lane];
+        kf = gld(a.pre_cnt + (item - a.pre_base));
+        if (lane < (int)kf) L->nd[lane] = gld(a.pre_nbrs + (size_t)(item - a.pre_base) * k + lane);
     } else {
         const KnnView v{a.W,     a.H,     a.tiles_x, a.tiles_y, a.mask, a.rlist, a.offtab, a.offtab_n, nullptr,
                         nullptr, nullptr, 0u,        a.gmask,   a.xl,   a.xr,    a.yb,     a.yt,       a.pend_r,
@@ -939,8 +977,8 @@ __device__ __forceinline__ uint4 item_at_random(const BatchArgs& a, const MapDes
     Pcg32 g2 = Pcg32::seed_from_u64(a.stage_seed);
     uint32_t ry = range_u32(g2, 0, maps[rm].h);
     const uint4 r = make_uint4(rx | (ry << 16), rm | (rm << 16) | kFallbackBit, flat, __float_as_uint(0.0f));
-    if (a.result) a.result[rel] = r;
-    if (a.item_score) a.item_score[item] = -1.0f;
+    if (a.result) gst(a.result + rel, r);
+    if (a.item_score) gst(a.item_score + item, -1.0f);
     if (a.dbg_best) a.dbg_best[rel] = 0xffffffffu;
     return r;
 }
@@ -1015,15 +1053,15 @@ __device__ __forceinline__ uint32_t item_neighbours(const BatchArgs& a, const Ma
         uint4 st;
         if (a.state2) {
             // deferred commit: the previous batch's records are still in state2
-            const uint32_t sp = a.stamp[nflat];
-            const uint4 s0 = a.state[nflat], s1 = a.state2[nflat];
+            const uint32_t sp = gld(a.stamp + nflat);
+            const uint4 s0 = gld(a.state + nflat), s1 = gld(a.state2 + nflat);
             st = sp == a.serial - 1u ? s1 : s0;
         } else {
             const uint4* src = (old_state && !((fresh >> lane) & 1ull)) ? old_state : a.state;
-            st = src[nflat];
+            st = gld(src + nflat);
         }
         L->pat[lane] = st.x;
-        if (GUIDED) L->gpat[lane] = a.tg[nflat];
+        if (GUIDED) L->gpat[lane] = gld(a.tg + nflat);
         if (!a.ext_cands) {
             int cx = (int)(st.y & 0xffffu) - packed_dx(o);
             int cy = (int)(st.y >> 16) - packed_dy(o);
@@ -1081,7 +1119,7 @@ __device__ __forceinline__ void item_exact(const BatchArgs& a, const MapDesc* ma
                 cmap = e.y;
                 sgn = 1;
             } else {
-                uint2 r = a.rcands[(size_t)(item - a.rc_base) * a.m + (c - n_coh)];
+                uint2 r = gld(a.rcands + (size_t)(item - a.rc_base) * a.m + (c - n_coh));
                 cxy = r.x;
                 cmap = r.y;
                 sgn = -1;
@@ -1094,7 +1132,7 @@ __device__ __forceinline__ void item_exact(const BatchArgs& a, const MapDesc* ma
                 const int X2 = cx + sgn * packed_dx(o), Y2 = cy + sgn * packed_dy(o);
                 const bool in = (uint32_t)X2 < md.w && (uint32_t)Y2 < md.h;
                 const uint32_t idx = (uint32_t)Y2 * md.w + (uint32_t)X2;
-                const uint32_t p = in ? md.px[idx] : kOutsidePx;
+                const uint32_t p = in ? gld(md.px + idx) : kOutsidePx;
                 constexpr int C = lut_copies<GUIDED>();
                 float s = lut4<C>(lut, bytes_scaled<C>(L->pat[j]), p, slot4);
                 if (GUIDED) {
@@ -1164,15 +1202,15 @@ __device__ __forceinline__ uint4 item_winner(const BatchArgs& a, const MapDesc* 
             cxy = xm->x;
             cmap = xm->y & 0x7fffffffu;
         } else {
-            const uint2 rc = a.rcands[(size_t)(item - a.rc_base) * a.m + (bidx - n_coh)];
+            const uint2 rc = gld(a.rcands + (size_t)(item - a.rc_base) * a.m + (bidx - n_coh));
             cxy = rc.x;
             cmap = rc.y;
         }
         patch = (cxy >> 16) * maps[cmap].w + (cxy & 0xffffu);
     }
     const uint4 r = make_uint4(cxy, cmap | (cmap << 16), patch, __float_as_uint(best));
-    if (a.result) a.result[rel] = r;
-    if (a.item_score) a.item_score[item] = best;
+    if (a.result) gst(a.result + rel, r);
+    if (a.item_score) gst(a.item_score + item, best);
     if (a.dbg_best) a.dbg_best[rel] = bidx;
     return r;
 }
@@ -1230,7 +1268,7 @@ __device__ __forceinline__ uint4 resolve_team(const BatchArgs& a, const MapDesc*
     uint2 rpre = make_uint2(0, 0);
     if (valid) {
         if (lead) phase_clock(a, item, 0, lane);
-        flat = a.work[item];
+        flat = gld(a.work + item);
         x = (int)(flat % a.W);
         y = (int)(flat / a.W);
         if (a.ext_cands) {
@@ -1247,7 +1285,7 @@ __device__ __forceinline__ uint4 resolve_team(const BatchArgs& a, const MapDesc*
         nrs = min(m_r, r0 + per) - r0;
         // the member's first 64 random candidates, loaded ahead of the kNN
         if (!a.ext_cands && !a.knn_only && (uint32_t)lane < nrs)
-            rpre = a.rcands[(size_t)(item - a.rc_base) * a.m + r0 + (uint32_t)lane];
+            rpre = gld(a.rcands + (size_t)(item - a.rc_base) * a.m + r0 + (uint32_t)lane);
     }
     // ---- 1. k nearest resolved neighbours (leader) ----
     // With a neighbour list computed ahead every member reads it and derives
@@ -1263,8 +1301,8 @@ __device__ __forceinline__ uint4 resolve_team(const BatchArgs& a, const MapDesc*
         if (pre && kf0) item_geometry(a, x, y, kf0, L, lane);
         if (flow) flow_wait(*flow, a, item, x, y, kf0, L, lane);
     } else if (pre) {
-        const uint32_t kf0 = a.pre_cnt[item - a.pre_base];
-        if (lane < (int)kf0) L->nd[lane] = a.pre_nbrs[(size_t)(item - a.pre_base) * a.k + lane];
+        const uint32_t kf0 = gld(a.pre_cnt + (item - a.pre_base));
+        if (lane < (int)kf0) L->nd[lane] = gld(a.pre_nbrs + (size_t)(item - a.pre_base) * a.k + lane);
         wave_sync();
         if (kf0) item_geometry(a, x, y, kf0, L, lane);
     }
@@ -1403,20 +1441,20 @@ __device__ __forceinline__ uint4 resolve_team(const BatchArgs& a, const MapDesc*
 __device__ __forceinline__ void commit_one(uint32_t W, uint32_t tiles_x, const MapDesc* maps, const uint32_t* work,
                                            const uint4* result, uint32_t item, uint32_t item0, uint32_t first_new,
                                            uint4* state, uint64_t* mask, float* score, uint64_t* gmask) {
-    uint32_t p = work[item];
-    uint4 r = result[item - item0];
+    uint32_t p = gld(work + item);
+    uint4 r = gld(result + (item - item0));
     uint32_t cmap = r.y & 0xffffu;
     const MapDesc md = maps[cmap];
     uint32_t sx = r.x & 0xffffu, sy = r.x >> 16;
-    uint32_t color = md.px[sy * md.w + sx];
-    state[p] = make_uint4(color, r.x, r.y & ~kFallbackBit, r.z);
+    uint32_t color = gld(md.px + sy * md.w + sx);
+    gst(state + p, make_uint4(color, r.x, r.y & ~kFallbackBit, r.z));
     if (item >= first_new) {
         uint32_t px = p % W, py = p / W;
         const uint32_t ti = (py >> 3) * tiles_x + (px >> 3);
         const uint64_t bit = 1ull << (((py & 7u) << 3) | (px & 7u));
         atomicOr((unsigned long long*)&mask[ti], bit);
         if (gmask && !(r.y & kFallbackBit)) atomicOr((unsigned long long*)&gmask[ti], bit);
-        score[p] = __uint_as_float(r.w);
+        gst(score + p, __uint_as_float(r.w));
     }
 }
 
@@ -1482,7 +1520,7 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve(BatchArgs a) {
             // the score is read only after the run
             const uint32_t p = a.work[item];
             const MapDesc& md = maps[r.y & 0xffffu];
-            const uint32_t color = md.px[(r.x >> 16) * md.w + (r.x & 0xffffu)];
+            const uint32_t color = gld(md.px + (r.x >> 16) * md.w + (r.x & 0xffffu));
             a.state2[p] = make_uint4(color, r.x, r.y & ~kFallbackBit, r.z);
             a.stamp[p] = a.serial;
             if (item >= a.first_new) {
@@ -1669,7 +1707,7 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve_flow(BatchArgs a
             // Generator::update (ms.rs:334-377): the state record, published
             // at once; the mask bit and the score are read only after the run
             const MapDesc& md = maps[r.y & 0xffffu];
-            const uint32_t color = md.px[(r.x >> 16) * md.w + (r.x & 0xffffu)];
+            const uint32_t color = gld(md.px + (r.x >> 16) * md.w + (r.x & 0xffffu));
             a.state_w[p] = make_uint4(color, r.x, r.y & ~kFallbackBit, r.z);
             __hip_atomic_store(f.flag + p, f.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             if (!f.state_old) {
@@ -1689,7 +1727,7 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) void k_resolve_flow(BatchArgs a
 // wavefront per item (the resolve kernels read them instead of searching).
 __device__ __forceinline__ void prefetch_item(const PrefetchArgs& a, WaveLds* L, uint32_t item, int lane) {
     const uint32_t rel = item - a.item0;
-    const uint32_t flat = a.work[item];
+    const uint32_t flat = gld(a.work + item);
     uint32_t n_at, bs = 0;
     if (a.n_at) {
         n_at = a.n_at[rel];
@@ -1707,8 +1745,8 @@ __device__ __forceinline__ void prefetch_item(const PrefetchArgs& a, WaveLds* L,
                     a.mask_new, a.tile_min, a.rank, n_at};
     uint32_t iters, buffered;
     const uint32_t kf = knn_search(v, L, (int)(flat % a.W), (int)(flat / a.W), a.k, n_at, lane, iters, buffered);
-    if (lane < (int)kf) a.out_nbrs[(size_t)rel * a.k + lane] = L->nd[lane];
-    if (lane == 0) a.out_cnt[rel] = kf;
+    if (lane < (int)kf) gst(a.out_nbrs + (size_t)rel * a.k + lane, L->nd[lane]);
+    if (lane == 0) gst(a.out_cnt + rel, kf);
     if (a.out_dep) {
         bool d = false;
         if (lane < (int)kf) {
