@@ -102,7 +102,11 @@ __device__ __forceinline__ void gst(uint4* p, uint4 v) {
     *(__attribute__((address_space(1))) u32x4*)p = w;
 }
 
-__device__ __forceinline__ uint64_t lanemask_lt(int lane) { return (1ull << lane) - 1ull; }
+// Set bits of m below the calling lane (v_mbcnt_lo, v_mbcnt_hi): no 64-bit
+// lane mask is held in registers.
+__device__ __forceinline__ uint32_t bits_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
 
 // Canonical neighbour order: squared distance, then dy, then dx, ascending.
 __device__ __forceinline__ uint64_t make_key(int dx, int dy) {
@@ -241,7 +245,7 @@ __device__ __noinline__ Sel sel_compact(uint64_t* keys, Sel s, uint32_t k, int l
         for (int r = 0; r < kKeyCap / kWave; ++r) {
             const bool sel = (uint32_t)(v[r] >> 32) <= lo;
             const uint64_t m = __ballot(sel);
-            if (sel) keys[base + __popcll(m & lanemask_lt(lane))] = v[r];
+            if (sel) keys[base + bits_below(m)] = v[r];
             base += (uint32_t)__popcll(m);
         }
         wave_sync();
@@ -263,7 +267,7 @@ __device__ __noinline__ Sel sel_compact(uint64_t* keys, Sel s, uint32_t k, int l
 __device__ __forceinline__ void sel_push(uint64_t* keys, Sel& s, uint32_t k, bool valid, uint64_t key, int lane) {
     bool take = valid && key < s.kth;
     uint64_t m = __ballot(take);
-    if (take) keys[s.count + __popcll(m & lanemask_lt(lane))] = key;
+    if (take) keys[s.count + bits_below(m)] = key;
     s.count += (uint32_t)__popcll(m);
     if (s.count > kKeyCap - kWave) {
         wave_sync();
@@ -581,7 +585,7 @@ __device__ __forceinline__ uint32_t knn_table(const KnnView& a, WaveLds* L, int 
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const uint64_t m = __ballot(hit[u]);
-            const uint32_t pre = (uint32_t)__popcll(m & lanemask_lt(lane));
+            const uint32_t pre = bits_below(m);
             if (hit[u] && found + pre < k) L->nd[found + pre] = o[u];
             found += (uint32_t)__popcll(m);
         }
@@ -901,7 +905,7 @@ __device__ __forceinline__ void eval_group(const MapDesc* maps, uint32_t n_maps,
         prune_bound(tb, best, PB, PBi);
         const bool keep = act && !(s > PB || (s == PB && (PBi == kNone || myidx > PBi)));
         const uint64_t km = __ballot(keep);
-        const uint32_t pos = (uint32_t)__popcll(km & lanemask_lt(lane));
+        const uint32_t pos = bits_below(km);
         wave_sync();
         if (keep) {
             L->act_xy[pos] = xy;
@@ -1071,7 +1075,7 @@ __device__ __forceinline__ uint32_t item_neighbours(const BatchArgs& a, const Ma
         }
     }
     uint64_t cm = __ballot(coh_valid);
-    if (coh_valid) L->coh[__popcll(cm & lanemask_lt(lane))] = coh;
+    if (coh_valid) L->coh[bits_below(cm)] = coh;
     const uint32_t n_coh = (uint32_t)__popcll(cm);
     wave_sync();
     if (!geo_done) item_geometry(a, x, y, kf, L, lane);
@@ -1346,7 +1350,7 @@ __device__ __forceinline__ uint4 resolve_team(const BatchArgs& a, const MapDesc*
             const uint32_t c0 = min(n_u, (uint32_t)member * cper);
             const uint32_t ncs = min(n_u, c0 + cper) - c0;
             const uint32_t nv = ncs + nrs;
-            const uint32_t upos = (uint32_t)__popcll(uniq & lanemask_lt(lane));
+            const uint32_t upos = bits_below(uniq);
             const bool mine_u = ((uniq >> lane) & 1ull) && upos >= c0 && upos < c0 + ncs;
             Best bst{FLT_MAX, kNone, 0u, 0u};
             uint32_t g_lo = 0;
