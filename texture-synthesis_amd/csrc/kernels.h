@@ -336,6 +336,10 @@ static_assert(sizeof(QCtl) == 256 * 9, "job table offsets, then one block per XC
 // For launches with fewer jobs than XCDs, which home placement would leave
 // on a few XCDs.
 constexpr uint32_t kEngineSpread = 1u;
+// The engine's first evaluation chunk while a pruning bound exists, as a
+// shift: 8 neighbours. With many jobs in flight the VALU binds, and pruning
+// from the first 8 terms saves more than the extra gather round trip costs.
+constexpr int kEngineFirstChunk = 3;
 
 struct QEngineArgs {
     const QJob* jobs;

@@ -795,7 +795,12 @@ __device__ __forceinline__ void prune_bound(const unsigned long long* tb, const 
     }
 }
 
-template <bool GUIDED>
+// FIRST_SH: while a pruning bound exists, the first chunk holds 2^FIRST_SH
+// neighbours. 3 (8 neighbours) starts pruning early, which pays where the
+// VALU binds (the multi-job engine); 6 (a full chunk) saves a gather round
+// trip, which pays where latency binds (the single-job kernels). A constant
+// here also lets the compiler keep the loop in fewer registers.
+template <bool GUIDED, int FIRST_SH>
 __device__ __forceinline__ void eval_group(const MapDesc* maps, uint32_t n_maps, WaveLds* L, const float* lut, const float* glut,
                                            int slot, int lane, uint32_t kf, uint32_t n_act, Best& best,
                                            unsigned long long* tb) {
@@ -810,8 +815,8 @@ __device__ __forceinline__ void eval_group(const MapDesc* maps, uint32_t n_maps,
     uint32_t j0 = 0;
     while (n_act > 0) {
         // chunk of 2^sh neighbours with n_act << sh <= kTermCap; while a bound
-        // exists the first chunk is kept at 8 neighbours so pruning starts early
-        uint32_t sh = (j0 == 0 && PBi != kNone) ? 3 : 6;
+        // exists the first chunk holds 2^FIRST_SH
+        uint32_t sh = (j0 == 0 && PBi != kNone) ? (uint32_t)FIRST_SH : 6u;
         while (sh > 0 && (n_act << sh) > (uint32_t)kTermCap) --sh;
         const uint32_t jc = 1u << sh;
         const uint32_t jn = min(jc, kf - j0);
@@ -1256,7 +1261,7 @@ __device__ __forceinline__ void flow_wait(const FlowArgs& f, const BatchArgs& a,
 }
 
 // Returns the result in the leader's lane 0 (zero elsewhere).
-template <bool GUIDED>
+template <bool GUIDED, int FIRST_SH = 6>
 __device__ __forceinline__ uint4 resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t item, uint32_t item0, uint32_t n_res,
                               WaveLds* Lt, int member, int T, bool valid, const float* lut, const float* glut,
                               int lane, const FlowArgs* flow = nullptr) {
@@ -1390,7 +1395,8 @@ __device__ __forceinline__ uint4 resolve_team(const BatchArgs& a, const MapDesc*
                     L->act_idx[lane] = ci;
                 }
                 wave_sync();
-                eval_group<GUIDED>(maps, a.n_maps, L, lut, glut, slot, lane, kf, g_hi - g_lo, bst, T > 1 ? &Lt->t_bound : nullptr);
+                eval_group<GUIDED, FIRST_SH>(maps, a.n_maps, L, lut, glut, slot, lane, kf, g_hi - g_lo, bst,
+                                             T > 1 ? &Lt->t_bound : nullptr);
                 if (lead && first) phase_clock(a, item, 4, lane);
                 first = false;
                 g_lo = g_hi;
@@ -2307,7 +2313,8 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) __attribute__((amdgpu_waves_per
                 const uint32_t item = P.i0 + (unit * P.rep + r) * per + (uint32_t)team;
                 int ln = lane;
                 asm volatile("" : "+v"(ln));
-                resolve_team<GUIDED>(a, maps, item, a.item0, a.n_res, wls + team * T, member, T, item < a.item1,
+                resolve_team<GUIDED, kEngineFirstChunk>(a, maps, item, a.item0, a.n_res, wls + team * T, member, T,
+                                                        item < a.item1,
                                      lut, glut, ln);
             }
             if (reload && tid == 0) s_key = key;
