@@ -364,6 +364,12 @@ def counters_from_profiles(name, kernel):
     return out or None
 
 
+def note(msg):
+    """A progress line on stderr as each leg ends (stdout carries only the
+    result line)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def limiter(c):
     """The limiter the counters point at."""
     if not c:
@@ -513,24 +519,32 @@ def main():
         "kernels_ms": {k: round(v["ms"], 3) for k, v in ks.items()},
         "device_ms_per_step": round(g.run_ms, 3),
     }
+    note("plain 1024^2: %.3f Mpix/s" % value)
     if args.guided:
         out["guided"] = guided_leg(ts, synth, args.guided)
+        note("guided: %.3f Mpix/s" % out["guided"]["value"])
     if args.plain2048:
         out["plain_2048"] = plain_2048_leg(ts, synth, args.plain2048)
+        note("plain 2048^2: %.3f Mpix/s" % out["plain_2048"]["value"])
     if args.batched:
         out["batched"] = batched_leg(ts, synth, args.batched, args.sequential)
         out["batched"]["vs_plain"] = round(out["batched"]["value"] / value, 2)
+        note("batched: %.3f Mpix/s" % out["batched"]["value"])
     if args.serve_requests:
         out["serving"] = serving_leg(ts, synth, args.serve_requests, args.serve_qps, args.serve_budget_gib)
+        note("serving: %d requests" % args.serve_requests)
     if args.serve_tight_requests:
         out["serving_tight_budget"] = serving_leg(ts, synth, args.serve_tight_requests, 30.0, args.serve_tight_gib)
+        note("serving, tight budget: %d requests" % args.serve_tight_requests)
     if args.serve_loaded_requests:
         out["serving_loaded"] = serving_leg(ts, synth, args.serve_loaded_requests, args.serve_loaded_qps,
                                             args.serve_budget_gib)
+        note("serving, loaded: %d requests" % args.serve_loaded_requests)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, host_cpu()[1])
         out["cpu_baseline"] = cpu_baseline(ex, threads)
         out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+        note("cpu baseline: %.3f Mpix/s" % out["cpu_baseline"]["value"])
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
