@@ -86,6 +86,26 @@ def test_outside_the_device_limits(ctx, setter, value):
         getattr(builder(), setter)(value).build()
 
 
+def test_examples_beyond_the_gather_offsets(ctx):
+    # gathers address an example level by a 32-bit byte offset, so an example
+    # (or example guide) holds at most 2^30 pixels; the sizes are checked
+    # before any pixel is read, so the descriptors point at a small buffer
+    L = ts.lib()
+    buf = np.zeros(64, np.uint8)
+    p = ts.Params()
+    L.tsg_params_default(C.byref(p))
+    job = C.c_void_p()
+    big = (ts.Image * 1)(ts.Image(32768, 32769, buf.ctypes.data))
+    rc = L.tsg_job_create(ctx._h, C.byref(p), 1, big, None, None, None, None, None, 0, 16, 16, C.byref(job))
+    assert rc == -11 and not job.value
+    ex = np.ascontiguousarray(synth.example(9, 16, 16))
+    tg = np.ascontiguousarray(synth.blobs(9, 16, 16))
+    exs = (ts.Image * 1)(ts.Image(16, 16, ex.ctypes.data))
+    tgi = ts.Image(16, 16, tg.ctypes.data)
+    rc = L.tsg_job_create(ctx._h, C.byref(p), 1, exs, None, None, big, C.byref(tgi), None, 0, 16, 16, C.byref(job))
+    assert rc == -11 and not job.value
+
+
 def test_random_init_beyond_the_pixels(ctx):
     with pytest.raises(ts.TsgError):
         builder(out=4).random_init(17).build()

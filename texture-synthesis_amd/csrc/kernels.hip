@@ -82,6 +82,16 @@ template <class T>
 __device__ __forceinline__ void gst(T* p, typename same_type<T>::type v) {
     *(__attribute__((address_space(1))) T*)p = v;
 }
+// Element of a device buffer at a 32-bit byte offset: with a uniform base
+// the load takes the scalar-base form, and the offset needs no 64-bit
+// arithmetic. Offsets fit: job creation bounds the output at 16384^2 (per
+// pixel tables of 4 bytes, tile masks of 2^22 words) and an example level at
+// 2^30 pixels.
+template <class T>
+__device__ __forceinline__ T gld_at(const T* base, uint32_t byte_off) {
+    return *(const __attribute__((address_space(1))) T*)(reinterpret_cast<const char*>(base) + byte_off);
+}
+
 // HIP's vector structs move as the native vectors they hold.
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -276,7 +286,7 @@ __device__ __forceinline__ void sel_push(uint64_t* keys, Sel& s, uint32_t k, boo
 }
 
 __device__ __forceinline__ bool px_set(const uint64_t* mask, uint32_t tiles_x, int X, int Y) {
-    uint64_t wd = gld(mask + (uint32_t)(Y >> 3) * tiles_x + (uint32_t)(X >> 3));
+    uint64_t wd = gld_at(mask, ((uint32_t)(Y >> 3) * tiles_x + (uint32_t)(X >> 3)) * 8u);
     return (wd >> (((Y & 7) << 3) | (X & 7))) & 1ull;
 }
 
@@ -315,8 +325,8 @@ __device__ __forceinline__ int32_t* offtab_lds() {
 
 // The resolved bits of tile ti.
 __device__ __forceinline__ uint64_t mask_word(const KnnView& v, uint32_t ti) {
-    uint64_t w = gld(v.mask + ti);
-    if (v.pend) w |= gld(v.pend + ti);
+    uint64_t w = gld_at(v.mask, ti * 8u);
+    if (v.pend) w |= gld_at(v.pend, ti * 8u);
     return w;
 }
 
@@ -579,7 +589,7 @@ __device__ __forceinline__ uint32_t knn_table(const KnnView& a, WaveLds* L, int 
                 const bool in = X >= 0 && Y >= 0 && X < (int)a.W && Y < (int)a.H;
                 hit[u] = in ? view_set(a, X, Y) : point_at(a, X, Y);
                 if (in && !hit[u] && a.mask_new && px_set(a.mask_new, a.tiles_x, X, Y))
-                    hit[u] = gld(a.rank + (uint32_t)Y * a.W + (uint32_t)X) < a.n_at;
+                    hit[u] = gld_at(a.rank, (__umul24((uint32_t)Y, a.W) + (uint32_t)X) * 4u) < a.n_at;
             }
         }
 #pragma unroll
@@ -720,14 +730,16 @@ __device__ __forceinline__ void gather_md(const MapDesc& md, uint32_t xy, uint32
     const uint32_t w = md.w, h = md.h;
     const bool mir = (ms >> 31) != 0;
     const int X2 = (int)(xy & 0xffffu) + (mir ? -dx : dx), Y2 = (int)(xy >> 16) + (mir ? -dy : dy);
+    // in bounds, Y2 and w are below 2^16 and a level holds at most 2^30
+    // pixels (checked at job creation), so the byte offset fits 32 bits
     const bool in = (uint32_t)X2 < w && (uint32_t)Y2 < h;
-    p = in ? gld(px + (uint32_t)Y2 * w + (uint32_t)X2) : kOutsidePx;
+    p = in ? gld_at(px, (__umul24((uint32_t)Y2, w) + (uint32_t)X2) * 4u) : kOutsidePx;
     gp = kOutsidePx;
     if (GUIDED) {
         const uint32_t* gpx = md.gpx;
         const uint32_t gw = md.gw, gh = md.gh;
         const bool gin = (uint32_t)X2 < gw && (uint32_t)Y2 < gh;
-        gp = gin ? gld(gpx + (uint32_t)Y2 * gw + (uint32_t)X2) : kOutsidePx;
+        gp = gin ? gld_at(gpx, (__umul24((uint32_t)Y2, gw) + (uint32_t)X2) * 4u) : kOutsidePx;
     }
 }
 template <bool GUIDED>

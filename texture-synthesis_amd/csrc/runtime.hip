@@ -2606,6 +2606,9 @@ int tsg_job_create(tsg_ctx* ctx, const tsg_params* prm, uint32_t n_examples, con
         if (!valid_image(&examples[i])) return fail(TSG_ERR_INVALID_ARGUMENT, "invalid example image");
         if (examples[i].w > 65535 || examples[i].h > 65535)
             return fail(TSG_ERR_UNSUPPORTED, "example side above 65535");
+        // the gathers address an example level by a 32-bit byte offset
+        if ((uint64_t)examples[i].w * examples[i].h > (1ull << 30))
+            return fail(TSG_ERR_UNSUPPORTED, "example above 2^30 pixels");
         int kd = sample_kind ? sample_kind[i] : TSG_SAMPLE_ALL;
         if (kd == TSG_SAMPLE_IMAGE) {
             if (!sample_masks || !valid_image(&sample_masks[i]))
@@ -2615,6 +2618,8 @@ int tsg_job_create(tsg_ctx* ctx, const tsg_params* prm, uint32_t n_examples, con
         }
         if (example_guides && !valid_image(&example_guides[i]))
             return fail(TSG_ERR_INVALID_ARGUMENT, "invalid example guide");
+        if (example_guides && (uint64_t)example_guides[i].w * example_guides[i].h > (1ull << 30))
+            return fail(TSG_ERR_UNSUPPORTED, "example guide above 2^30 pixels");
     }
     if (target_guide && (!valid_image(target_guide) || target_guide->w != out_w || target_guide->h != out_h))
         return fail(TSG_ERR_SIZE_MISMATCH, "target guide must have the output size");
