@@ -340,6 +340,10 @@ constexpr uint32_t kEngineSpread = 1u;
 // shift: 8 neighbours. With many jobs in flight the VALU binds, and pruning
 // from the first 8 terms saves more than the extra gather round trip costs.
 constexpr int kEngineFirstChunk = 3;
+// The engine's offset-table kNN: offsets per lane per round trip (64 per
+// round). A dense mask holds k neighbours within the first 64 or so, and the
+// VALU binds, so rounds are kept short.
+constexpr int kEngineKnnU = 1;
 
 struct QEngineArgs {
     const QJob* jobs;

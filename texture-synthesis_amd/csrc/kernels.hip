@@ -571,20 +571,23 @@ __device__ __forceinline__ void knn_rings(const KnnView& a, WaveLds* L, int x, i
 
 // Ring-ordered offset table scan: the first k resolved pixels in table
 // order are the k nearest (the table lists every offset within its radius
-// in canonical order). 256 offsets per memory round trip, taken in table
-// order. Returns the number found (<= k).
+// in canonical order). KU x 64 offsets per memory round trip, taken in table
+// order: 4 (256) where latency binds; fewer where the VALU binds, since a
+// dense mask holds k neighbours within the first 64 or so. Returns the
+// number found (<= k).
+template <int KU>
 __device__ __forceinline__ uint32_t knn_table(const KnnView& a, WaveLds* L, int x, int y, uint32_t k, int lane) {
     uint32_t found = 0;
-    for (uint32_t base = 0; base < a.offtab_n && found < k; base += 4 * kWave) {
-        int32_t o[4];
-        bool hit[4];
+    for (uint32_t base = 0; base < a.offtab_n && found < k; base += KU * kWave) {
+        int32_t o[KU];
+        bool hit[KU];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < KU; ++u) {
             const uint32_t t = base + u * kWave + lane;
             o[u] = 0;
             hit[u] = false;
             if (t < a.offtab_n) {
-                o[u] = (a.offc && base < kOffCache) ? a.offc[t] : gld(a.offtab + t);
+                o[u] = (a.offc && t < kOffCache) ? a.offc[t] : gld(a.offtab + t);
                 const int X = x + packed_dx(o[u]), Y = y + packed_dy(o[u]);
                 const bool in = X >= 0 && Y >= 0 && X < (int)a.W && Y < (int)a.H;
                 hit[u] = in ? view_set(a, X, Y) : point_at(a, X, Y);
@@ -593,7 +596,7 @@ __device__ __forceinline__ uint32_t knn_table(const KnnView& a, WaveLds* L, int 
             }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < KU; ++u) {
             const uint64_t m = __ballot(hit[u]);
             const uint32_t pre = bits_below(m);
             if (hit[u] && found + pre < k) L->nd[found + pre] = o[u];
@@ -604,7 +607,9 @@ __device__ __forceinline__ uint32_t knn_table(const KnnView& a, WaveLds* L, int 
 }
 
 // k nearest of (x, y) among the view's resolved pixels (n_res of them), in
-// canonical order, into L->nd; returns the count (<= k).
+// canonical order, into L->nd; returns the count (<= k). KU: the offset
+// table's offsets per lane per round trip.
+template <int KU = 4>
 __device__ __forceinline__ uint32_t knn_search(const KnnView& v, WaveLds* L, int x, int y, uint32_t k, uint32_t n_res,
                                                int lane, uint32_t& iters, uint32_t& buffered) {
     uint32_t found = 0;
@@ -613,7 +618,7 @@ __device__ __forceinline__ uint32_t knn_search(const KnnView& v, WaveLds* L, int
     // round trips) reach k resolved pixels; sparser masks use the ring search
     const bool sparse = (uint64_t)n_res * 1024ull < (uint64_t)k * v.W * v.H;
     if (!use_sel && !sparse) {
-        found = knn_table(v, L, x, y, k, lane);
+        found = knn_table<KU>(v, L, x, y, k, lane);
         // fewer than k within the table: search on, unless every point was
         // found (without ghosts there are n_res of them)
         use_sel = found < k && (v.gmask || found < n_res);
@@ -957,6 +962,7 @@ __device__ __forceinline__ void phase_value(const BatchArgs& a, uint32_t item, i
 
 // Leader, step 1: kNN into L->nd; returns the count. Test hooks: host lists,
 // dumps, knn_only.
+template <int KU>
 __device__ __forceinline__ uint32_t item_knn(const BatchArgs& a, uint32_t item, uint32_t rel, int x, int y,
                                              uint32_t n_res, WaveLds* L, int lane) {
     const uint32_t k = a.k;
@@ -972,7 +978,7 @@ __device__ __forceinline__ uint32_t item_knn(const BatchArgs& a, uint32_t item, 
                         nullptr, nullptr, 0u,        a.gmask,   a.xl,   a.xr,    a.yb,     a.yt,       a.pend_r,
                         offtab_lds()};
         uint32_t iters, buffered;
-        kf = knn_search(v, L, x, y, k, n_res, lane, iters, buffered);
+        kf = knn_search<KU>(v, L, x, y, k, n_res, lane, iters, buffered);
         phase_clock(a, item, 7, lane);
         phase_value(a, item, 8, iters, lane);
         phase_value(a, item, 9, buffered, lane);
@@ -1273,7 +1279,7 @@ __device__ __forceinline__ void flow_wait(const FlowArgs& f, const BatchArgs& a,
 }
 
 // Returns the result in the leader's lane 0 (zero elsewhere).
-template <bool GUIDED, int FIRST_SH = 6>
+template <bool GUIDED, int FIRST_SH = 6, int KNN_U = 4>
 __device__ __forceinline__ uint4 resolve_team(const BatchArgs& a, const MapDesc* maps, uint32_t item, uint32_t item0, uint32_t n_res,
                               WaveLds* Lt, int member, int T, bool valid, const float* lut, const float* glut,
                               int lane, const FlowArgs* flow = nullptr) {
@@ -1314,7 +1320,7 @@ __device__ __forceinline__ uint4 resolve_team(const BatchArgs& a, const MapDesc*
     // neighbours committed inside the run).
     const bool pre = valid && !a.ext_nbrs && !a.knn_only && a.pre_nbrs && item >= a.pre_base && item < a.pre_end;
     if (valid && lead) {
-        const uint32_t kf0 = item_knn(a, item, rel, x, y, n_res, L, lane);
+        const uint32_t kf0 = item_knn<KNN_U>(a, item, rel, x, y, n_res, L, lane);
         if (lane == 0) {
             L->t_kf = kf0;
             L->t_bound = ~0ull;
@@ -2325,9 +2331,8 @@ __global__ __launch_bounds__(kWave* kWavesPerWG) __attribute__((amdgpu_waves_per
                 const uint32_t item = P.i0 + (unit * P.rep + r) * per + (uint32_t)team;
                 int ln = lane;
                 asm volatile("" : "+v"(ln));
-                resolve_team<GUIDED, kEngineFirstChunk>(a, maps, item, a.item0, a.n_res, wls + team * T, member, T,
-                                                        item < a.item1,
-                                     lut, glut, ln);
+                resolve_team<GUIDED, kEngineFirstChunk, kEngineKnnU>(a, maps, item, a.item0, a.n_res, wls + team * T,
+                                                                     member, T, item < a.item1, lut, glut, ln);
             }
             if (reload && tid == 0) s_key = key;
         } else if (P.kind == Q_COMMIT) {
