@@ -726,15 +726,15 @@ __device__ __forceinline__ void cand_get(const BatchArgs& a, const WaveLds* L, u
 // Gather of a candidate's pattern pixel for neighbour j (mirrored for
 // random candidates, ms.rs:534/588): colour and, when guided, guide pixel;
 // out-of-bounds pixels are [0,0,0,255] (get_color_of_neighbor, ms.rs:1122-1149).
-// The descriptor's fields are read together, ahead of the bounds test, so
-// that their loads are not chained behind one another.
+// Pattern pixel of a candidate at xy for a neighbour at (dx, dy), the
+// offset already negated for a mirrored pattern. The descriptor's fields are
+// read together, ahead of the bounds test, so that their loads are not
+// chained behind one another.
 template <bool GUIDED>
-__device__ __forceinline__ void gather_md(const MapDesc& md, uint32_t xy, uint32_t ms, int dx, int dy,
-                                          uint32_t& p, uint32_t& gp) {
+__device__ __forceinline__ void gather_xy(const MapDesc& md, uint32_t xy, int dx, int dy, uint32_t& p, uint32_t& gp) {
     const uint32_t* px = md.px;
     const uint32_t w = md.w, h = md.h;
-    const bool mir = (ms >> 31) != 0;
-    const int X2 = (int)(xy & 0xffffu) + (mir ? -dx : dx), Y2 = (int)(xy >> 16) + (mir ? -dy : dy);
+    const int X2 = (int)(xy & 0xffffu) + dx, Y2 = (int)(xy >> 16) + dy;
     // in bounds, Y2 and w are below 2^16 and a level holds at most 2^30
     // pixels (checked at job creation), so the byte offset fits 32 bits
     const bool in = (uint32_t)X2 < w && (uint32_t)Y2 < h;
@@ -747,10 +747,12 @@ __device__ __forceinline__ void gather_md(const MapDesc& md, uint32_t xy, uint32
         gp = gin ? gld_at(gpx, (__umul24((uint32_t)Y2, gw) + (uint32_t)X2) * 4u) : kOutsidePx;
     }
 }
+// Candidate (xy, map word ms): its map, and its mirror bit applied per lane.
 template <bool GUIDED>
 __device__ __forceinline__ void gather_at(const MapDesc* maps, uint32_t xy, uint32_t ms, int dx, int dy,
                                           uint32_t& p, uint32_t& gp) {
-    gather_md<GUIDED>(maps[ms & 0x7fffffffu], xy, ms, dx, dy, p, gp);
+    const bool mir = (ms >> 31) != 0;
+    gather_xy<GUIDED>(maps[ms & 0x7fffffffu], xy, mir ? -dx : dx, mir ? -dy : dy, p, gp);
 }
 
 // Target side of neighbour j, the same for every candidate: its pattern and
@@ -826,6 +828,12 @@ __device__ __forceinline__ void eval_group(const MapDesc* maps, uint32_t n_maps,
     if (n_maps == 1) m0 = ld_uniform(maps);
     if (lane < (int)n_act) L->act_sum[lane] = 0.0f;
     wave_sync();
+    // the group's mirror state: 0 none, 1 all, 2 mixed. A group holds
+    // coherence candidates only (never mirrored) or random ones only (always
+    // mirrored); host-given candidate lists may mix them.
+    const uint64_t g_all = __ballot(lane < (int)n_act);
+    const uint64_t g_mm = __ballot(lane < (int)n_act && (L->act_ms[lane] >> 31) != 0);
+    const int g_mir = g_mm == 0 ? 0 : (g_mm == g_all ? 1 : 2);
     float PB;
     uint32_t PBi;
     prune_bound(tb, best, PB, PBi);
@@ -844,14 +852,16 @@ __device__ __forceinline__ void eval_group(const MapDesc* maps, uint32_t n_maps,
         const bool jin = jj < jn;
         const NbrRow row = nbr_row<GUIDED>(L, j0 + (jin ? jj : 0u));
         // all gathers of the chunk first (at most kTermCap / 64 per lane), then
-        // the LUT sums; a single example's descriptor is held in scalars
+        // the LUT sums. With one example (its descriptor held in scalars) and
+        // one mirror state for the group, the offset's sign is applied once.
         uint32_t px[kTermCap / kWave], gpx[kTermCap / kWave];
-        if (n_maps == 1) {
+        if (n_maps == 1 && g_mir != 2) {
+            const int ddx = g_mir ? -row.dx : row.dx, ddy = g_mir ? -row.dy : row.dy;
 #pragma unroll
             for (int u = 0; u < kTermCap / kWave; ++u) {
                 const uint32_t l = (uint32_t)(u * kWave + lane), ai = l >> sh;
                 px[u] = gpx[u] = 0;
-                if (l < total && jin) gather_md<GUIDED>(m0, L->act_xy[ai], L->act_ms[ai], row.dx, row.dy, px[u], gpx[u]);
+                if (l < total && jin) gather_xy<GUIDED>(m0, L->act_xy[ai], ddx, ddy, px[u], gpx[u]);
             }
         } else {
 #pragma unroll
